@@ -133,6 +133,66 @@ __device__ __forceinline__ T hypot_c(T x, T y)
 template <typename T>
 __device__ __forceinline__ T hypot1_c(T x) { return hypot_c(x, T(1)); }
 
+// 1/sqrt(x) from the hardware estimate plus Newton steps y <- y*(3 - x*y*y)/2:
+// one step for float (estimate ~1 ulp -> ~0.5 ulp + rounding), two for double
+// (estimate ~2^-22 -> full precision).  x must be positive and normal.
+__device__ __forceinline__ float rsqrt_nr(float x)
+{
+  float y = __builtin_amdgcn_rsqf(x);
+  const float hx = 0.5f * x;
+  y = y * __builtin_fmaf(-hx * y, y, 1.5f);
+  return y;
+}
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * __builtin_fma(-hx * y, y, 1.5);
+  y = y * __builtin_fma(-hx * y, y, 1.5);
+  return y;
+}
+
+// Givens parameters for (f, g): r = sqrt(f^2 + g^2), s = f/r, c = g/r.
+// Returns false (and r = 0) when f = g = 0.
+template <typename T>
+__device__ __forceinline__ bool givens(T f, T g, T &r, T &c, T &s)
+{
+  const T ss = fma_(f, f, g * g);
+  if (ss == T(0)) { r = T(0); return false; }
+  const T ri = rsqrt_nr(ss);
+  r = ss * ri;
+  c = g * ri;
+  s = f * ri;
+  return true;
+}
+
+// ---- one value per lane of a group: element i lives in lane i ------------
+// Group-uniform index i.  For a full wave the element is read with
+// v_readlane (the index is wave-uniform, so it sits in an SGPR); for smaller
+// groups with ds_bpermute.
+template <int NB>
+__device__ __forceinline__ float gread(float v, int i)
+{
+  if (NB == 64) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+  const int base = (int)(threadIdx.x & 63u) & ~(NB - 1);
+  return __int_as_float(__builtin_amdgcn_ds_bpermute((base + i) << 2, __float_as_int(v)));
+}
+template <int NB>
+__device__ __forceinline__ int gread(int v, int i)
+{
+  if (NB == 64) return __builtin_amdgcn_readlane(v, i);
+  const int base = (int)(threadIdx.x & 63u) & ~(NB - 1);
+  return __builtin_amdgcn_ds_bpermute((base + i) << 2, v);
+}
+template <int NB>
+__device__ __forceinline__ double gread(double v, int i)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = gread<NB>((int)(b & 0xffffffffll), i);
+  const int hi = gread<NB>((int)(b >> 32), i);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // 8 consecutive T from LDS with 16-byte loads (p must be 16-byte aligned).
 __device__ __forceinline__ void ld8(const float *p, float o[8])
 {

@@ -36,21 +36,18 @@ __host__ __device__ constexpr int uoff(int c)
 
 template <typename T, int NB>
 struct alignas(16) WaveLds {
-  alignas(16) T U[uoff(NB)];              // packed Householder vectors
-  alignas(16) T beta[NB];                 // beta_c = 1/(u_l * anorm)
-  alignas(16) T rowbuf[NB];               // row c of A (= column c)
-  alignas(16) T vbuf[NB];                 // v of the current step
-  alignas(16) T d[NB];                    // diagonal, then eigenvalues
-  alignas(16) T e[NB];                    // e[c] couples c and c+1, e[n-1] = 0
-  alignas(16) typename vec2<T>::type cs[NB]; // (c, s) of pair (i, i+1)
-  alignas(16) int pos[NB];                // output slot of register column j
+  alignas(16) T U[uoff(NB)];                 // packed Householder vectors
+  alignas(16) T rowbuf[NB];                  // row c of A (= column c)
+  alignas(16) T vbuf[NB];                    // v of the current step
+  alignas(16) typename vec2<T>::type cs[NB]; // (c, s) of pair (i, i+1) in one QL sweep
+  alignas(16) int pos[NB];                   // output slot of register column j
 };
 
 template <typename T, int NB>
 __device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const WaveLds<T, NB> &S, int lo, int hi)
 {
-  // Pairs (i, i+1) for i in [lo, hi] (possibly identity outside the sweep's own
-  // range), applied from the top: h1 carries the pending value of column i+1.
+  // Pairs (i, i+1) for i in [lo, hi], identity outside the sweep's own range,
+  // applied from the top: h1 carries the pending value of column i+1.
   const int blo = lo >> 3, bhi = hi >> 3;
   T h1 = T(0);
 #pragma unroll
@@ -70,6 +67,10 @@ __device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const WaveLds
   }
 }
 
+// Set element i (group-uniform) of a lane-distributed vector.
+template <typename T>
+__device__ __forceinline__ T gset(T v, int k, int i, T x) { return (k == i) ? x : v; }
+
 template <typename T, int NB>
 __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *__restrict__ a,
                                                       T *__restrict__ w, int *__restrict__ info,
@@ -87,14 +88,22 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
   WaveLds<T, NB> &S = smem[g];
   T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
 
+  // R holds row k of A during the reduction and row k of Z afterwards.
+  T R[NB];
   // ---- load: row k of the symmetric matrix from the upper triangle ----------
-  T Ar[NB];
+  {
+    const int kc = (k < n) ? k : n - 1;
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    T v = T(0);
-    if (j < n && k < n) v = (k <= j) ? A[k + (size_t)j * nm] : A[j + (size_t)k * nm];
-    Ar[j] = v;
+    for (int j = 0; j < NB; ++j) {
+      const int jc = (j < n) ? j : n - 1;
+      const size_t idx = (kc <= jc) ? (size_t)kc + (size_t)jc * nm : (size_t)jc + (size_t)kc * nm;
+      const T v = A[idx];
+      R[j] = (j < n && k < n) ? v : T(0);
+    }
   }
+
+  // d, e and beta are lane-distributed: element c lives in lane c of the group.
+  T dreg = T(0), ereg = T(0), breg = T(0);
 
   // ---- Householder tridiagonalisation (tred1_tiled.hpp:42-212) --------------
   for (int c = n - 1; c >= 1; --c) {
@@ -102,7 +111,7 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
     if (k == c) {
 #pragma unroll
       for (int b = 0; b < NBLK; ++b)
-        if (b < nbr) st8(&S.rowbuf[8 * b], &Ar[8 * b]);
+        if (b < nbr) st8(&S.rowbuf[8 * b], &R[8 * b]);
     }
     wsync();
     T uk = (k < c) ? S.rowbuf[k] : T(0);
@@ -116,7 +125,9 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
     const int off = uoff(c);
     const int nbc = (c + 7) >> 3;           // 8-blocks covering columns 0..c-1
     if (k < 8 * nbc) S.U[off + k] = uk;
-    if (k == 0) { S.e[c - 1] = anorm; S.d[c] = dc; S.beta[c] = beta; }
+    dreg = gset(dreg, k, c, dc);
+    ereg = gset(ereg, k, c - 1, anorm);
+    breg = gset(breg, k, c, beta);
     wsync();
     // p = beta * A(0:c-1, 0:c-1) u  (row k of the product)
     T p = T(0);
@@ -126,7 +137,7 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         T u8[8];
         ld8(&S.U[off + 8 * b], u8);
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) p = fma_(Ar[8 * b + jj], u8[jj], p);
+        for (int jj = 0; jj < 8; ++jj) p = fma_(R[8 * b + jj], u8[jj], p);
       }
     }
     p = (k < c) ? p * beta : T(0);
@@ -142,22 +153,20 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         ld8(&S.U[off + 8 * b], u8);
         ld8(&S.vbuf[8 * b], v8);
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) Ar[8 * b + jj] = fma_(uk, v8[jj], fma_(vk, u8[jj], Ar[8 * b + jj]));
+        for (int jj = 0; jj < 8; ++jj) R[8 * b + jj] = fma_(uk, v8[jj], fma_(vk, u8[jj], R[8 * b + jj]));
       }
     }
     wsync();
   }
-  if (k == 0) { S.d[0] = Ar[0]; S.e[n - 1] = T(0); }
+  dreg = gset(dreg, k, 0, R[0]);
 
   // ---- Q = H_{n-1} ... H_1 (0-based creation order), right-multiplied -------
-  T Zr[NB];
 #pragma unroll
-  for (int j = 0; j < NB; ++j) Zr[j] = (j == k) ? T(1) : T(0);
-  wsync();
+  for (int j = 0; j < NB; ++j) R[j] = (j == k) ? T(1) : T(0);
   for (int c = n - 1; c >= 1; --c) {
     const int off = uoff(c);
     const int nbc = (c + 7) >> 3;
-    const T beta = S.beta[c];
+    const T beta = gread<NB>(breg, c);
     T t = T(0);
 #pragma unroll
     for (int b = 0; b < NBLK; ++b) {
@@ -165,7 +174,7 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         T u8[8];
         ld8(&S.U[off + 8 * b], u8);
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) t = fma_(Zr[8 * b + jj], u8[jj], t);
+        for (int jj = 0; jj < 8; ++jj) t = fma_(R[8 * b + jj], u8[jj], t);
       }
     }
     t *= beta;
@@ -175,73 +184,75 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         T u8[8];
         ld8(&S.U[off + 8 * b], u8);
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) Zr[8 * b + jj] = fma_(t, u8[jj], Zr[8 * b + jj]);
+        for (int jj = 0; jj < 8; ++jj) R[8 * b + jj] = fma_(t, u8[jj], R[8 * b + jj]);
       }
     }
   }
 
-  // ---- QL with eigenvectors accumulated into Z ------------------------------
+  // ---- QL with eigenvectors accumulated into Z = R --------------------------
+  // (c, s) of the current sweep: lane i holds the pair (i, i+1); identity
+  // outside the sweep.  Dumped to LDS once per sweep for the rotation chain.
   const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
-  S.cs[k] = T2{T(1), T(0)};
-  wsync();
+  T csc = T(1), css = T(0);
   int err = 0;
   if (solver == SOLVER_TQL2) {
     // explicit shift (tql2_tiled.hpp:40-146)
     T shift = T(0), tst1 = T(0);
     for (int l = 0; l < n && !err; ++l) {
-      const T h = (abs_(S.d[l]) + abs_(S.e[l])) * tol;
+      const T h = (abs_(gread<NB>(dreg, l)) + abs_(gread<NB>(ereg, l))) * tol;
       tst1 = (h > tst1) ? h : tst1;
-      const bool cand = (k >= l && k <= n - 1) && (abs_(S.e[k < n ? k : 0]) <= tst1);
+      const bool cand = (k >= l && k <= n - 1) && (abs_(ereg) <= tst1);
       const uint64_t msk = group_ballot<NB>(cand);
       const int m = msk ? __builtin_ctzll(msk) : n - 1;
       if (m != l) {
         int itr;
         for (itr = 0; itr < QL_MAX_SWEEP; ++itr) {
-          const T dl_old = S.d[l], el = S.e[l];
-          const T p0 = (S.d[l + 1] - dl_old) / (el + el);
+          const T dl_old = gread<NB>(dreg, l), el = gread<NB>(ereg, l);
+          const T p0 = (gread<NB>(dreg, l + 1) - dl_old) / (el + el);
           const T r0 = hypot1_c(p0);
           const T psr = p0 + sign_(r0, p0);
           const T dl = el / psr, dl1 = el * psr;
           const T delta = dl_old - dl;
-          wsync();
-          if (k == 0) { S.d[l] = dl; S.d[l + 1] = dl1; }
-          if (k >= l + 2 && k < n) S.d[k] -= delta;
+          dreg = (k == l) ? dl : (k == l + 1) ? dl1 : (k >= l + 2 && k < n) ? dreg - delta : dreg;
           shift += delta;
-          wsync();
           T c = T(1), c2 = T(1), c3 = T(1), s = T(0), s2 = T(0);
-          T pp = S.d[m];
-          const T el1 = S.e[l + 1];
-          T ei = S.e[m - 1], di = S.d[m - 1];
+          T pp = gread<NB>(dreg, m);
+          const T el1 = gread<NB>(ereg, l + 1);
+          T ei = gread<NB>(ereg, m - 1), di = gread<NB>(dreg, m - 1);
           for (int i = m - 1; i >= l; --i) {
-            const T ein = (i > l) ? S.e[i - 1] : T(0);
-            const T din = (i > l) ? S.d[i - 1] : T(0);
+            const int ip = (i > 0) ? i - 1 : 0;
+            const T ein = gread<NB>(ereg, ip), din = gread<NB>(dreg, ip);
             c3 = c2; c2 = c; s2 = s;
             const T gg = c * ei, hh = c * pp;
-            const T r = hypot_c(pp, ei);
+            T r, cn, sn;
+            if (!givens(ei, pp, r, cn, sn)) { cn = T(1); sn = T(0); }
             const T ei1 = s * r;
-            s = ei / r;
-            c = pp / r;
+            s = sn;
+            c = cn;
             pp = c * di - s * gg;
             const T di1 = hh + s * (c * gg + s * di);
-            if (k == 0) { S.e[i + 1] = ei1; S.d[i + 1] = di1; S.cs[i] = T2{c, s}; }
-            if (i > l) { ei = ein; di = din; }
+            ereg = gset(ereg, k, i + 1, ei1);
+            dreg = gset(dreg, k, i + 1, di1);
+            csc = gset(csc, k, i, c);
+            css = gset(css, k, i, s);
+            ei = ein;
+            di = din;
           }
-          const T rr = (-s * s2 * c3 * el1 * ei) / dl1;
+          const T rr = (-s * s2 * c3 * el1 * el) / dl1;
           const T el_new = s * rr;
-          if (k == 0) { S.e[l] = el_new; S.d[l] = c * rr; }
+          ereg = gset(ereg, k, l, el_new);
+          dreg = gset(dreg, k, l, c * rr);
+          S.cs[k] = T2{csc, css};
           wsync();
-          apply_sweep_rotations<T, NB>(Zr, S, l, m - 1);
+          apply_sweep_rotations<T, NB>(R, S, l, m - 1);
           wsync();
-          if (k >= l && k <= m - 1) S.cs[k] = T2{T(1), T(0)};
-          wsync();
+          csc = T(1);
+          css = T(0);
           if (abs_(el_new) <= tst1) break;
         }
         if (itr >= QL_MAX_SWEEP) err = l + 1;
       }
-      if (!err) {
-        if (k == 0) S.d[l] += shift;
-        wsync();
-      }
+      if (!err) dreg = (k == l) ? dreg + shift : dreg;
     }
   } else {
     // implicit shift (imtql2_tiled.hpp:235-321)
@@ -250,52 +261,54 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         // itr sweeps done; the reference stops after max_sweep+1 sweeps without
         // a final test (imtql2_tiled.hpp:242,319)
         if (itr > QL_MAX_SWEEP) { err = l + 1; break; }
-        bool cand = false;
-        if (k >= l && k <= n - 2)
-          cand = abs_(S.e[k]) <= (abs_(S.d[k]) + abs_(S.d[k + 1])) * tol;
+        const T dk1 = lane_bcast(dreg, lane + 1);
+        const bool cand = (k >= l && k <= n - 2) && abs_(ereg) <= (abs_(dreg) + abs_(dk1)) * tol;
         const uint64_t msk = group_ballot<NB>(cand);
         const int m = msk ? __builtin_ctzll(msk) : n - 1;
         if (m == l) break;
-        const T dl = S.d[l], el = S.e[l];
-        const T f0 = (S.d[l + 1] - dl) / (el + el);
+        const T dl = gread<NB>(dreg, l), el = gread<NB>(ereg, l);
+        const T f0 = (gread<NB>(dreg, l + 1) - dl) / (el + el);
         const T g0 = hypot1_c(f0);
-        T di1 = S.d[m];
+        T di1 = gread<NB>(dreg, m);
         T r = (di1 - dl) + el / (f0 + sign_(g0, f0));
         T s = T(1), c = T(1), delta = T(0);
-        T ei = S.e[m - 1], di = S.d[m - 1];
+        T ei = gread<NB>(ereg, m - 1), di = gread<NB>(dreg, m - 1);
         int i;
         for (i = m - 1; i >= l; --i) {
-          const T ein = (i > l) ? S.e[i - 1] : T(0);
-          const T din = (i > l) ? S.d[i - 1] : T(0);
+          const int ip = (i > 0) ? i - 1 : 0;
+          const T ein = gread<NB>(ereg, ip), din = gread<NB>(dreg, ip);
           const T f = s * ei, gg = r;
-          r = hypot_c(f, gg);
-          if (r == T(0)) break;
-          const T bb = c * ei;
-          s = f / r;
-          c = gg / r;
+          const T bb = c * ei;                   // old c
+          if (!givens(f, gg, r, c, s)) break;    // r = 0: underflow, restart the sweep
           const T dix = di1 - delta;
           di1 = di;
           const T q = (di1 - dix) * s + T(2) * c * bb;
           const T dx1 = fma_(s, q, dix);
           delta = dx1 - dix;
-          if (k == 0) { S.e[i + 1] = r; S.d[i + 1] = dx1; S.cs[i] = T2{c, s}; }
+          ereg = gset(ereg, k, i + 1, r);
+          dreg = gset(dreg, k, i + 1, dx1);
+          csc = gset(csc, k, i, c);
+          css = gset(css, k, i, s);
           r = c * q - bb;
           ei = ein;
           di = din;
         }
         const int j = (i + 1 > l) ? i + 1 : l;
+        dreg = (k == j) ? dreg - delta : dreg;
+        ereg = gset(ereg, k, j, r);
+        ereg = gset(ereg, k, m, T(0));
+        S.cs[k] = T2{csc, css};
         wsync();
-        if (k == 0) { S.d[j] -= delta; S.e[j] = r; S.e[m] = T(0); }
-        apply_sweep_rotations<T, NB>(Zr, S, l, m - 1);
+        apply_sweep_rotations<T, NB>(R, S, l, m - 1);
         wsync();
-        if (k >= l && k <= m - 1) S.cs[k] = T2{T(1), T(0)};
-        wsync();
+        csc = T(1);
+        css = T(0);
       }
     }
   }
 
   if (err) {
-    if (k < n) w[(size_t)mat * n + k] = S.d[k];
+    if (k < n) w[(size_t)mat * n + k] = dreg;
     if (info && k == 0) info[mat] = err;
     return;
   }
@@ -303,7 +316,7 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
 
   // ---- ascending sort, the reference's selection rule (tql2_tiled.hpp:149-176):
   // step l swaps slot l with the first minimum of slots l..n-1.
-  T dk = (k < n) ? S.d[k] : T(0);
+  T dk = dreg;
   int pk = k;
   const int base = lane & ~(NB - 1);
   for (int l = 0; l < n - 1; ++l) {
@@ -336,7 +349,7 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
   for (int j = 0; j < NB; ++j) {
     if (j < n) {
       const int r = S.pos[j];
-      if (k < n) A[k + (size_t)r * nm] = Zr[j];
+      if (k < n) A[k + (size_t)r * nm] = R[j];
     }
   }
 }
