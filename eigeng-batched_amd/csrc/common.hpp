@@ -153,13 +153,27 @@ __device__ __forceinline__ double rsqrt_nr(double x)
 }
 
 // Givens parameters for (f, g): r = sqrt(f^2 + g^2), s = f/r, c = g/r.
-// Returns false (and r = 0) when f = g = 0.
-template <typename T>
-__device__ __forceinline__ bool givens(T f, T g, T &r, T &c, T &s)
+// Returns false (and r = 0) when f = g = 0.  The float version renormalises
+// (c, s) once with the fma-exact residual e = c^2 + s^2 - 1, so c^2 + s^2 - 1
+// is O(eps^2) plus final rounding: no bias accumulates in the eigenvectors.
+__device__ __forceinline__ bool givens(float f, float g, float &r, float &c, float &s)
 {
-  const T ss = fma_(f, f, g * g);
-  if (ss == T(0)) { r = T(0); return false; }
-  const T ri = rsqrt_nr(ss);
+  const float ss = __builtin_fmaf(f, f, g * g);
+  if (ss == 0.0f) { r = 0.0f; return false; }
+  const float ri = rsqrt_nr(ss);
+  r = ss * ri;
+  c = g * ri;
+  s = f * ri;
+  const float h = -0.5f * __builtin_fmaf(c, c, __builtin_fmaf(s, s, -1.0f));
+  c = __builtin_fmaf(c, h, c);
+  s = __builtin_fmaf(s, h, s);
+  return true;
+}
+__device__ __forceinline__ bool givens(double f, double g, double &r, double &c, double &s)
+{
+  const double ss = __builtin_fma(f, f, g * g);
+  if (ss == 0.0) { r = 0.0; return false; }
+  const double ri = rsqrt_nr(ss);
   r = ss * ri;
   c = g * ri;
   s = f * ri;

@@ -91,14 +91,33 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
   // R holds row k of A during the reduction and row k of Z afterwards.
   T R[NB];
   // ---- load: row k of the symmetric matrix from the upper triangle ----------
+  // Columns j > k come from row k of the upper triangle (coalesced across
+  // lanes); columns j <= k come from column k (a(j,k), j <= k, is the upper
+  // triangle too), read by each lane as one contiguous run with 16-byte loads.
   {
     const int kc = (k < n) ? k : n - 1;
+    constexpr int V = 16 / sizeof(T);      // elements per 16-byte load
+    typedef T vload_t __attribute__((ext_vector_type(V), aligned(sizeof(T))));
+    const T *colk = A + (size_t)kc * nm;
+#pragma unroll
+    for (int q = 0; q < NB / V; ++q) {
+      T v[V];
+      if (V * q + V <= n) {
+        const vload_t x = *(const vload_t *)(colk + V * q);
+#pragma unroll
+        for (int t = 0; t < V; ++t) v[t] = x[t];
+      } else {
+#pragma unroll
+        for (int t = 0; t < V; ++t) v[t] = (V * q + t < n) ? colk[V * q + t] : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < V; ++t) R[V * q + t] = v[t];
+    }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       const int jc = (j < n) ? j : n - 1;
-      const size_t idx = (kc <= jc) ? (size_t)kc + (size_t)jc * nm : (size_t)jc + (size_t)kc * nm;
-      const T v = A[idx];
-      R[j] = (j < n && k < n) ? v : T(0);
+      const T v = A[(size_t)kc + (size_t)jc * nm];
+      R[j] = (j < n && k < n) ? ((j > k) ? v : R[j]) : T(0);
     }
   }
 
