@@ -1,24 +1,30 @@
-// eig_wave.hpp - one matrix per group of NB lanes (n <= NB <= 64), whole
-// pipeline in one kernel: Householder tridiagonalisation -> explicit
-// Q = H_n ... H_2 -> QL with rotations accumulated into Q -> ascending sort ->
-// permuted store of the eigenvectors.
+// eig_wave.hpp - register-resident solver for n <= NB <= 128: one matrix per
+// group of NB lanes (NB = 8..64: a group inside one wave64; NB = 128: two
+// waves), whole pipeline in one kernel: Householder tridiagonalisation ->
+// explicit Q = H_n ... H_2 -> QL with the rotations accumulated into Q ->
+// ascending sort -> permuted store of the eigenvectors.
 //
-// Layout (per group): lane k owns row k of the matrix in registers (A[NB], then
-// Z[NB]; column index = register index, so every loop over columns is fully
-// unrolled and column ranges are skipped in uniform blocks of 8).  The
-// Householder vectors go to LDS packed by column (column c padded to a multiple
-// of 8 with zeros) and are read back as 16-byte broadcasts; d, e, the rotation
-// pairs (c, s) of one QL sweep and the sort permutation live in LDS too.
-// Applying H_c from the right (Z <- Z H_c) and a Givens rotation of columns
-// (i, i+1) are both lane-local in this layout, so no cross-lane traffic is
-// needed outside the reductions of the tridiagonalisation.
+// Layout: lane k of the group owns row k of the matrix in registers (R[NB],
+// column index = register index, so every loop over columns is fully unrolled
+// and column ranges are skipped in uniform blocks of 8).  R holds A during the
+// reduction and Z afterwards.  The Householder vectors go to LDS packed by
+// column (column c padded to a multiple of 8 with zeros) and are read back as
+// 16-byte broadcasts.  Applying H_c from the right (Z <- Z H_c) and a Givens
+// rotation of columns (i, i+1) are both lane-local in this layout.
+//
+// The tridiagonal (d, e), the betas and the (c, s) of the current QL sweep are
+// lane-distributed vectors: element i lives in lane i % 64 of every wave of the
+// group (slot i / 64), replicated in each wave, so the QL phase runs with no
+// cross-wave synchronisation at all; for a single wave, element i is read with
+// v_readlane (wave-uniform index in an SGPR), otherwise with ds_bpermute.
 //
 // Reference path replaced: eigen_GPU_batch_tiled_ (eigen_GPU_batch.cu:56-135)
 // with tred1_tiled_ (tred1_tiled.hpp:3-222), imtql2_tiled_/tql2_tiled_
 // (imtql2_tiled.hpp:3-177, tql2_tiled.hpp:3-192) and hhtr2sy_tiled_
-// (hhtr2sy_tiled.hpp:3-226).  The reference back-transforms after QL
-// (Z <- H_2 ... H_n Z_QL); here Q = H_n ... H_2 is formed first and the QL
-// rotations are accumulated into it, which gives the same product Q * Z_QL.
+// (hhtr2sy_tiled.hpp:3-226); for 64 < n <= 128 the reference's wave kernel
+// eigen_GPU_batch_ (eigen_GPU_batch.cu:138-245).  The reference back-transforms
+// after QL (Z <- H_2 ... H_n Z_QL); here Q = H_n ... H_2 is formed first and the
+// QL rotations are accumulated into it, which gives the same product Q * Z_QL.
 #pragma once
 #include "common.hpp"
 
@@ -34,17 +40,61 @@ __host__ __device__ constexpr int uoff(int c)
   return 8 * ((c - 1) / 8 + 1) * (4 * ((c - 1) / 8) + (c - 1) % 8);
 }
 
-template <typename T, int NB>
-struct alignas(16) WaveLds {
-  alignas(16) T U[uoff(NB)];                 // packed Householder vectors
-  alignas(16) T rowbuf[NB];                  // row c of A (= column c)
-  alignas(16) T vbuf[NB];                    // v of the current step
-  alignas(16) typename vec2<T>::type cs[NB]; // (c, s) of pair (i, i+1) in one QL sweep
-  alignas(16) int pos[NB];                   // output slot of register column j
+template <int NB> struct GroupShape {
+  static constexpr int NL = NB < 64 ? NB : 64; // lanes of the group inside one wave
+  static constexpr int NW = NB / NL;           // waves per group
+  static constexpr int G = 64 / NL;            // groups per wave
 };
 
 template <typename T, int NB>
-__device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const WaveLds<T, NB> &S, int lo, int hi)
+struct alignas(16) WaveLds {
+  static constexpr int NW = GroupShape<NB>::NW;
+  alignas(16) T U[uoff(NB)];                     // packed Householder vectors
+  alignas(16) T rowbuf[NB];                      // row c of A (= column c)
+  alignas(16) T vbuf[NB];                        // v of the current step
+  alignas(16) T red[2][2];                       // cross-wave partial sums
+  alignas(16) typename vec2<T>::type cs[NW][NB]; // per wave: (c, s) of pair (i, i+1)
+  alignas(16) int pos[NW][NB];                   // per wave: output slot of column j
+};
+
+template <int NW>
+__device__ __forceinline__ void gsync()
+{
+  if (NW == 1) wsync();
+  else __syncthreads();
+}
+
+// Sum over the NB lanes of the group; `slot` selects one of two scratch pairs
+// (two consecutive reductions must use different slots or be separated by a
+// group barrier).
+template <int NB, typename T>
+__device__ __forceinline__ T gsum(T x, T (&red)[2][2], int slot, int wv, int kl)
+{
+  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW;
+  x = group_sum<NL>(x);
+  if (NW == 1) return x;
+  if (kl == 0) red[slot][wv] = x;
+  __syncthreads();
+  return red[slot][0] + red[slot][1];
+}
+
+// Lane-distributed vectors: element i in slot i / NL of lane i % NL.
+template <int NL, int NW, typename T>
+__device__ __forceinline__ T vget(const T (&v)[NW], int i)
+{
+  if (NW == 1) return gread<NL>(v[0], i);
+  return (i < NL) ? gread<NL>(v[0], i) : gread<NL>(v[NW - 1], i - NL);
+}
+template <int NL, int NW, typename T>
+__device__ __forceinline__ void vset(T (&v)[NW], int kl, int i, T x)
+{
+#pragma unroll
+  for (int h = 0; h < NW; ++h) v[h] = (kl + h * NL == i) ? x : v[h];
+}
+
+template <typename T, int NB>
+__device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const typename vec2<T>::type *cs,
+                                                      int lo, int hi)
 {
   // Pairs (i, i+1) for i in [lo, hi], identity outside the sweep's own range,
   // applied from the top: h1 carries the pending value of column i+1.
@@ -57,7 +107,7 @@ __device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const WaveLds
       if (b == bhi) h1 = Zr[itop + 1];
 #pragma unroll
       for (int i = itop; i >= 8 * b; --i) {
-        const typename vec2<T>::type q = S.cs[i];
+        const typename vec2<T>::type q = cs[i];
         const T h0 = Zr[i];
         Zr[i + 1] = q.y * h0 + q.x * h1;
         h1 = q.x * h0 - q.y * h1;
@@ -67,24 +117,21 @@ __device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const WaveLds
   }
 }
 
-// Set element i (group-uniform) of a lane-distributed vector.
-template <typename T>
-__device__ __forceinline__ T gset(T v, int k, int i, T x) { return (k == i) ? x : v; }
-
 template <typename T, int NB>
-__global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *__restrict__ a,
-                                                      T *__restrict__ w, int *__restrict__ info,
-                                                      int solver)
+__global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
+    int L, int nm, int n, T *__restrict__ a, T *__restrict__ w, int *__restrict__ info, int solver)
 {
-  constexpr int G = 64 / NB;
+  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW, G = GroupShape<NB>::G;
   constexpr int NBLK = NB / 8;
   typedef typename vec2<T>::type T2;
   __shared__ WaveLds<T, NB> smem[G];
 
   const int lane = threadIdx.x & 63;
-  const int g = lane / NB, k = lane % NB;
+  const int wv = (int)(threadIdx.x >> 6);   // wave of the group (NW > 1)
+  const int g = lane / NL, kl = lane % NL;
+  const int k = wv * NL + kl;               // matrix row owned by this lane
   const long long mat = (long long)blockIdx.x * G + g;
-  if (mat >= L) return;  // whole group leaves together
+  if (mat >= L) return;  // whole group leaves together (only for NW == 1; otherwise G == 1)
   WaveLds<T, NB> &S = smem[g];
   T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
 
@@ -121,8 +168,9 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
     }
   }
 
-  // d, e and beta are lane-distributed: element c lives in lane c of the group.
-  T dreg = T(0), ereg = T(0), breg = T(0);
+  T dv[NW], ev[NW], bv[NW];
+#pragma unroll
+  for (int h = 0; h < NW; ++h) { dv[h] = T(0); ev[h] = T(0); bv[h] = T(0); }
 
   // ---- Householder tridiagonalisation (tred1_tiled.hpp:42-212) --------------
   for (int c = n - 1; c >= 1; --c) {
@@ -132,11 +180,11 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
       for (int b = 0; b < NBLK; ++b)
         if (b < nbr) st8(&S.rowbuf[8 * b], &R[8 * b]);
     }
-    wsync();
+    gsync<NW>();
     T uk = (k < c) ? S.rowbuf[k] : T(0);
     const T dc = S.rowbuf[c];
     const T ulast = S.rowbuf[c - 1];
-    const T sig = group_sum<NB>(uk * uk);
+    const T sig = gsum<NB>(uk * uk, S.red, 0, wv, kl);
     const T anorm = -sign_(sqrt_(sig), ulast);
     const T ul = ulast - anorm;
     const T beta = T(1) / nz(ul * anorm);
@@ -144,10 +192,10 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
     const int off = uoff(c);
     const int nbc = (c + 7) >> 3;           // 8-blocks covering columns 0..c-1
     if (k < 8 * nbc) S.U[off + k] = uk;
-    dreg = gset(dreg, k, c, dc);
-    ereg = gset(ereg, k, c - 1, anorm);
-    breg = gset(breg, k, c, beta);
-    wsync();
+    vset<NL, NW>(dv, kl, c, dc);
+    vset<NL, NW>(ev, kl, c - 1, anorm);
+    vset<NL, NW>(bv, kl, c, beta);
+    gsync<NW>();
     // p = beta * A(0:c-1, 0:c-1) u  (row k of the product)
     T p = T(0);
 #pragma unroll
@@ -160,10 +208,10 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
       }
     }
     p = (k < c) ? p * beta : T(0);
-    const T alpha = group_sum<NB>(p * uk) * (beta * T(0.5));
+    const T alpha = gsum<NB>(p * uk, S.red, 1, wv, kl) * (beta * T(0.5));
     const T vk = fma_(alpha, uk, p);
     S.vbuf[k] = vk;
-    wsync();
+    gsync<NW>();
     // A(0:c-1,0:c-1) += u v^T + v u^T
 #pragma unroll
     for (int b = 0; b < NBLK; ++b) {
@@ -175,9 +223,14 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         for (int jj = 0; jj < 8; ++jj) R[8 * b + jj] = fma_(uk, v8[jj], fma_(vk, u8[jj], R[8 * b + jj]));
       }
     }
-    wsync();
+    if (NW == 1) wsync();
   }
-  dreg = gset(dreg, k, 0, R[0]);
+  {
+    // d(0) = A(0,0) lives in row 0 (lane 0 of wave 0); broadcast to every wave
+    if (k == 0) S.rowbuf[0] = R[0];
+    gsync<NW>();
+    vset<NL, NW>(dv, kl, 0, S.rowbuf[0]);
+  }
 
   // ---- Q = H_{n-1} ... H_1 (0-based creation order), right-multiplied -------
 #pragma unroll
@@ -185,7 +238,7 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
   for (int c = n - 1; c >= 1; --c) {
     const int off = uoff(c);
     const int nbc = (c + 7) >> 3;
-    const T beta = gread<NB>(breg, c);
+    const T beta = vget<NL, NW>(bv, c);
     T t = T(0);
 #pragma unroll
     for (int b = 0; b < NBLK; ++b) {
@@ -209,38 +262,72 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
   }
 
   // ---- QL with eigenvectors accumulated into Z = R --------------------------
-  // (c, s) of the current sweep: lane i holds the pair (i, i+1); identity
-  // outside the sweep.  Dumped to LDS once per sweep for the rotation chain.
+  // Everything below is per wave (replicated vectors, a private (c, s) array):
+  // no cross-wave synchronisation.
+  T2 *const cs = S.cs[wv];
   const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
-  T csc = T(1), css = T(0);
+  T csc[NW], css[NW];
+#pragma unroll
+  for (int h = 0; h < NW; ++h) { csc[h] = T(1); css[h] = T(0); }
   int err = 0;
+
+  // first index m >= l with the predicate set (lane-distributed), else n - 1
+  auto first_set = [&](const bool (&cand)[NW]) -> int {
+    if (NW == 1) {
+      const uint64_t msk = group_ballot<NL>(cand[0]);
+      return msk ? __builtin_ctzll(msk) : n - 1;
+    }
+#pragma unroll
+    for (int h = 0; h < NW; ++h) {
+      const uint64_t msk = __ballot(cand[h]);
+      if (msk) return h * 64 + __builtin_ctzll(msk);
+    }
+    return n - 1;
+  };
+  auto dump_cs = [&]() {
+#pragma unroll
+    for (int h = 0; h < NW; ++h) cs[kl + h * NL] = T2{csc[h], css[h]};
+  };
+  auto reset_cs = [&]() {
+#pragma unroll
+    for (int h = 0; h < NW; ++h) { csc[h] = T(1); css[h] = T(0); }
+  };
+
   if (solver == SOLVER_TQL2) {
     // explicit shift (tql2_tiled.hpp:40-146)
     T shift = T(0), tst1 = T(0);
     for (int l = 0; l < n && !err; ++l) {
-      const T h = (abs_(gread<NB>(dreg, l)) + abs_(gread<NB>(ereg, l))) * tol;
+      const T h = (abs_(vget<NL, NW>(dv, l)) + abs_(vget<NL, NW>(ev, l))) * tol;
       tst1 = (h > tst1) ? h : tst1;
-      const bool cand = (k >= l && k <= n - 1) && (abs_(ereg) <= tst1);
-      const uint64_t msk = group_ballot<NB>(cand);
-      const int m = msk ? __builtin_ctzll(msk) : n - 1;
+      bool cand[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        const int idx = kl + q * NL;
+        cand[q] = (idx >= l && idx <= n - 1) && (abs_(ev[q]) <= tst1);
+      }
+      const int m = first_set(cand);
       if (m != l) {
         int itr;
         for (itr = 0; itr < QL_MAX_SWEEP; ++itr) {
-          const T dl_old = gread<NB>(dreg, l), el = gread<NB>(ereg, l);
-          const T p0 = (gread<NB>(dreg, l + 1) - dl_old) / (el + el);
+          const T dl_old = vget<NL, NW>(dv, l), el = vget<NL, NW>(ev, l);
+          const T p0 = (vget<NL, NW>(dv, l + 1) - dl_old) / (el + el);
           const T r0 = hypot1_c(p0);
           const T psr = p0 + sign_(r0, p0);
           const T dl = el / psr, dl1 = el * psr;
           const T delta = dl_old - dl;
-          dreg = (k == l) ? dl : (k == l + 1) ? dl1 : (k >= l + 2 && k < n) ? dreg - delta : dreg;
+#pragma unroll
+          for (int q = 0; q < NW; ++q) {
+            const int idx = kl + q * NL;
+            dv[q] = (idx == l) ? dl : (idx == l + 1) ? dl1 : (idx >= l + 2 && idx < n) ? dv[q] - delta : dv[q];
+          }
           shift += delta;
           T c = T(1), c2 = T(1), c3 = T(1), s = T(0), s2 = T(0);
-          T pp = gread<NB>(dreg, m);
-          const T el1 = gread<NB>(ereg, l + 1);
-          T ei = gread<NB>(ereg, m - 1), di = gread<NB>(dreg, m - 1);
+          T pp = vget<NL, NW>(dv, m);
+          const T el1 = vget<NL, NW>(ev, l + 1);
+          T ei = vget<NL, NW>(ev, m - 1), di = vget<NL, NW>(dv, m - 1);
           for (int i = m - 1; i >= l; --i) {
             const int ip = (i > 0) ? i - 1 : 0;
-            const T ein = gread<NB>(ereg, ip), din = gread<NB>(dreg, ip);
+            const T ein = vget<NL, NW>(ev, ip), din = vget<NL, NW>(dv, ip);
             c3 = c2; c2 = c; s2 = s;
             const T gg = c * ei, hh = c * pp;
             T r, cn, sn;
@@ -250,28 +337,30 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
             c = cn;
             pp = c * di - s * gg;
             const T di1 = hh + s * (c * gg + s * di);
-            ereg = gset(ereg, k, i + 1, ei1);
-            dreg = gset(dreg, k, i + 1, di1);
-            csc = gset(csc, k, i, c);
-            css = gset(css, k, i, s);
+            vset<NL, NW>(ev, kl, i + 1, ei1);
+            vset<NL, NW>(dv, kl, i + 1, di1);
+            vset<NL, NW>(csc, kl, i, c);
+            vset<NL, NW>(css, kl, i, s);
             ei = ein;
             di = din;
           }
           const T rr = (-s * s2 * c3 * el1 * el) / dl1;
           const T el_new = s * rr;
-          ereg = gset(ereg, k, l, el_new);
-          dreg = gset(dreg, k, l, c * rr);
-          S.cs[k] = T2{csc, css};
+          vset<NL, NW>(ev, kl, l, el_new);
+          vset<NL, NW>(dv, kl, l, c * rr);
+          dump_cs();
           wsync();
-          apply_sweep_rotations<T, NB>(R, S, l, m - 1);
+          apply_sweep_rotations<T, NB>(R, cs, l, m - 1);
           wsync();
-          csc = T(1);
-          css = T(0);
+          reset_cs();
           if (abs_(el_new) <= tst1) break;
         }
         if (itr >= QL_MAX_SWEEP) err = l + 1;
       }
-      if (!err) dreg = (k == l) ? dreg + shift : dreg;
+      if (!err) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) dv[q] = (kl + q * NL == l) ? dv[q] + shift : dv[q];
+      }
     }
   } else {
     // implicit shift (imtql2_tiled.hpp:235-321)
@@ -280,22 +369,31 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
         // itr sweeps done; the reference stops after max_sweep+1 sweeps without
         // a final test (imtql2_tiled.hpp:242,319)
         if (itr > QL_MAX_SWEEP) { err = l + 1; break; }
-        const T dk1 = lane_bcast(dreg, lane + 1);
-        const bool cand = (k >= l && k <= n - 2) && abs_(ereg) <= (abs_(dreg) + abs_(dk1)) * tol;
-        const uint64_t msk = group_ballot<NB>(cand);
-        const int m = msk ? __builtin_ctzll(msk) : n - 1;
+        bool cand[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          const int idx = kl + q * NL;
+          // d(idx+1): next lane of the same slot, or lane 0 of the next slot
+          T dn = lane_bcast(dv[q], lane + 1);
+          if (NW > 1) {
+            const T d0 = lane_bcast(dv[q + 1 < NW ? q + 1 : q], 0);
+            dn = (kl == NL - 1) ? d0 : dn;
+          }
+          cand[q] = (idx >= l && idx <= n - 2) && abs_(ev[q]) <= (abs_(dv[q]) + abs_(dn)) * tol;
+        }
+        const int m = first_set(cand);
         if (m == l) break;
-        const T dl = gread<NB>(dreg, l), el = gread<NB>(ereg, l);
-        const T f0 = (gread<NB>(dreg, l + 1) - dl) / (el + el);
+        const T dl = vget<NL, NW>(dv, l), el = vget<NL, NW>(ev, l);
+        const T f0 = (vget<NL, NW>(dv, l + 1) - dl) / (el + el);
         const T g0 = hypot1_c(f0);
-        T di1 = gread<NB>(dreg, m);
+        T di1 = vget<NL, NW>(dv, m);
         T r = (di1 - dl) + el / (f0 + sign_(g0, f0));
         T s = T(1), c = T(1), delta = T(0);
-        T ei = gread<NB>(ereg, m - 1), di = gread<NB>(dreg, m - 1);
+        T ei = vget<NL, NW>(ev, m - 1), di = vget<NL, NW>(dv, m - 1);
         int i;
         for (i = m - 1; i >= l; --i) {
           const int ip = (i > 0) ? i - 1 : 0;
-          const T ein = gread<NB>(ereg, ip), din = gread<NB>(dreg, ip);
+          const T ein = vget<NL, NW>(ev, ip), din = vget<NL, NW>(dv, ip);
           const T f = s * ei, gg = r;
           const T bb = c * ei;                   // old c
           if (!givens(f, gg, r, c, s)) break;    // r = 0: underflow, restart the sweep
@@ -304,30 +402,34 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
           const T q = (di1 - dix) * s + T(2) * c * bb;
           const T dx1 = fma_(s, q, dix);
           delta = dx1 - dix;
-          ereg = gset(ereg, k, i + 1, r);
-          dreg = gset(dreg, k, i + 1, dx1);
-          csc = gset(csc, k, i, c);
-          css = gset(css, k, i, s);
+          vset<NL, NW>(ev, kl, i + 1, r);
+          vset<NL, NW>(dv, kl, i + 1, dx1);
+          vset<NL, NW>(csc, kl, i, c);
+          vset<NL, NW>(css, kl, i, s);
           r = c * q - bb;
           ei = ein;
           di = din;
         }
         const int j = (i + 1 > l) ? i + 1 : l;
-        dreg = (k == j) ? dreg - delta : dreg;
-        ereg = gset(ereg, k, j, r);
-        ereg = gset(ereg, k, m, T(0));
-        S.cs[k] = T2{csc, css};
+#pragma unroll
+        for (int q = 0; q < NW; ++q) dv[q] = (kl + q * NL == j) ? dv[q] - delta : dv[q];
+        vset<NL, NW>(ev, kl, j, r);
+        vset<NL, NW>(ev, kl, m, T(0));
+        dump_cs();
         wsync();
-        apply_sweep_rotations<T, NB>(R, S, l, m - 1);
+        apply_sweep_rotations<T, NB>(R, cs, l, m - 1);
         wsync();
-        csc = T(1);
-        css = T(0);
+        reset_cs();
       }
     }
   }
 
   if (err) {
-    if (k < n) w[(size_t)mat * n + k] = dreg;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const int idx = kl + q * NL;
+      if (q == wv && idx < n) w[(size_t)mat * n + idx] = dv[q];
+    }
     if (info && k == 0) info[mat] = err;
     return;
   }
@@ -335,16 +437,24 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
 
   // ---- ascending sort, the reference's selection rule (tql2_tiled.hpp:149-176):
   // step l swaps slot l with the first minimum of slots l..n-1.
-  T dk = dreg;
-  int pk = k;
-  const int base = lane & ~(NB - 1);
-  for (int l = 0; l < n - 1; ++l) {
-    const bool in = (k >= l && k < n);
-    T v = dk;
-    int ix = in ? k : NB;
-    // argmin (value, index), ties to the smaller index; out-of-range lanes lose
+  int pv[NW];
 #pragma unroll
-    for (int sh = 1; sh < NB; sh <<= 1) {
+  for (int q = 0; q < NW; ++q) pv[q] = kl + q * NL;
+  for (int l = 0; l < n - 1; ++l) {
+    // this lane's best candidate over its slots (ties to the smaller index)
+    T v = T(0);
+    int ix = NB;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const int idx = kl + q * NL;
+      const bool in = (idx >= l && idx < n);
+      const bool take = in && (ix == NB || dv[q] < v);
+      v = take ? dv[q] : v;
+      ix = take ? idx : ix;
+    }
+    // argmin (value, index) over the lanes, ties to the smaller index
+#pragma unroll
+    for (int sh = 1; sh < NL; sh <<= 1) {
       const T v2 = lane_bcast(v, lane ^ sh);
       const int i2 = lane_bcast(ix, lane ^ sh);
       const bool take = (i2 < NB) && (ix == NB || v2 < v || (v2 == v && i2 < ix));
@@ -352,22 +462,29 @@ __global__ __launch_bounds__(64) void eig_wave_kernel(int L, int nm, int n, T *_
       ix = take ? i2 : ix;
     }
     if (ix != l) {
-      const T d_l = lane_bcast(dk, base + l);
-      const int p_l = lane_bcast(pk, base + l);
-      const int p_i = lane_bcast(pk, base + ix);
-      if (k == l) { dk = v; pk = p_i; }
-      if (k == ix) { dk = d_l; pk = p_l; }
+      const T d_l = vget<NL, NW>(dv, l);
+      const int p_l = vget<NL, NW>(pv, l);
+      const int p_i = vget<NL, NW>(pv, ix);
+      vset<NL, NW>(dv, kl, l, v);
+      vset<NL, NW>(pv, kl, l, p_i);
+      vset<NL, NW>(dv, kl, ix, d_l);
+      vset<NL, NW>(pv, kl, ix, p_l);
     }
   }
-  if (k < n) {
-    w[(size_t)mat * n + k] = dk;
-    S.pos[pk] = k;
+  int *const posw = S.pos[wv];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) {
+    const int idx = kl + q * NL;
+    if (idx < n) {
+      if (q == wv) w[(size_t)mat * n + idx] = dv[q];
+      posw[pv[q]] = idx;
+    }
   }
   wsync();
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (j < n) {
-      const int r = S.pos[j];
+      const int r = posw[j];
       if (k < n) A[k + (size_t)r * nm] = R[j];
     }
   }

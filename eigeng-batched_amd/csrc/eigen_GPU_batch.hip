@@ -25,17 +25,17 @@ bool bad_args(int L, int nm, int n, int m) { return L < 1 || nm < n || n < 1 || 
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
   (void)L; (void)nm; (void)m; (void)tbytes; (void)solver;
-  if (n <= 64) return 256;
+  if (n <= 128) return 256;
   return 0; // larger n: not supported by this build
 }
 
 template <typename T, int NB>
 hipError_t launch_wave(int L, int nm, int n, T *a, T *w, int *info, int solver, hipStream_t s)
 {
-  constexpr int G = 64 / NB;
+  constexpr int G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
   const long long blocks = ((long long)L + G - 1) / G;
   if (blocks > 0x7fffffffll) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3((unsigned)blocks), dim3(64), 0, s,
+  hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3((unsigned)blocks), dim3(64 * NW), 0, s,
                      L, nm, n, a, w, info, solver);
   return hipGetLastError();
 }
@@ -50,6 +50,7 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, solver, s);
   if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
   if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
+  if (n <= 128) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
   return hipErrorNotSupported;
 }
 

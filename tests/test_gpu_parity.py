@@ -65,7 +65,8 @@ def check_batch(a0, z, w, dtype, wo=None, zo=None, vec_check=True):
             assert not bad.any(), f"matrix {k}: eigenvector mismatch at {np.nonzero(bad)[0][:5]}"
 
 
-SIZES = [1, 2, 3, 4, 5, 7, 8, 9, 12, 15, 16, 17, 24, 31, 32, 33, 47, 48, 49, 63, 64]
+SIZES = [1, 2, 3, 4, 5, 7, 8, 9, 12, 15, 16, 17, 24, 31, 32, 33, 47, 48, 49, 63, 64,
+         65, 66, 96, 100, 127, 128]
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
@@ -86,7 +87,7 @@ def test_sym_rand_vs_oracle(n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [5, 32, 64])
+@pytest.mark.parametrize("n", [5, 32, 64, 100])
 def test_tql2_variant(n, dtype):
     print(f"[parity] tql2 n={n} dtype={np.dtype(dtype).name}", flush=True)
     a0 = wl.sym_rand(19, n, n, dtype=dtype, seed=77 + n)
@@ -97,7 +98,7 @@ def test_tql2_variant(n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [3, 10, 32, 48, 64])
+@pytest.mark.parametrize("n", [3, 10, 32, 48, 64, 97, 128])
 def test_frank_closed_form(n, dtype):
     print(f"[parity] frank n={n} dtype={np.dtype(dtype).name}", flush=True)
     nm = n + (n & 1)

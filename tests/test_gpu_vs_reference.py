@@ -58,7 +58,7 @@ def run_mine(a_np):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [3, 5, 8, 16, 17, 31, 32, 33, 48, 63, 64])
+@pytest.mark.parametrize("n", [3, 5, 8, 16, 17, 31, 32, 33, 48, 63, 64, 65, 96, 128])
 def test_matches_reference_eigenpairs(reflib, n, dtype):
     print(f"[vs-ref] n={n} dtype={np.dtype(dtype).name}", flush=True)
     nm = n + (n & 1)
