@@ -12,6 +12,7 @@
 
 #include "../../include/eigen_GPU_batch.h"
 #include "eig_wave.hpp"
+#include "eig_global.hpp"
 
 using namespace eigmi;
 
@@ -19,14 +20,25 @@ namespace {
 
 bool bad_args(int L, int nm, int n, int m) { return L < 1 || nm < n || n < 1 || m < 1 || m > n; }
 
-// Workspace: the n <= 64 kernels keep everything in registers and LDS; the
-// workspace is a small fixed block so that the reference's "wk == NULL means
-// query" protocol always yields a non-NULL allocation.
+constexpr int WS_HEAD = 256;      // bytes; keeps the workspace non-empty and aligned
+
+// Which kernel serves (n, element size): register-resident groups while a row
+// of the matrix fits in the register file, the workgroup/global-memory kernel
+// beyond.
+bool use_wave_kernel(int n, int tbytes) { return tbytes == 4 ? n <= 128 : n <= 64; }
+
+int global_grid(int L) { return L < 1024 ? L : 1024; }
+
+// Workspace: the register kernels keep everything in registers and LDS (a
+// small fixed block keeps the reference's "wk == NULL means query" protocol
+// yielding a non-NULL allocation); the global kernel needs one n x n
+// accumulator per resident workgroup.
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
-  (void)L; (void)nm; (void)m; (void)tbytes; (void)solver;
-  if (n <= 128) return 256;
-  return 0; // larger n: not supported by this build
+  (void)nm; (void)m; (void)solver;
+  if (use_wave_kernel(n, tbytes)) return WS_HEAD;
+  if (n > GL_NMAX) return 0;
+  return WS_HEAD + (size_t)global_grid(L) * (size_t)n * (size_t)n * (size_t)tbytes;
 }
 
 template <typename T, int NB>
@@ -41,6 +53,15 @@ hipError_t launch_wave(int L, int nm, int n, T *a, T *w, int *info, int solver, 
 }
 
 template <typename T>
+hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
+{
+  T *zws = (T *)((char *)wk + WS_HEAD);
+  hipLaunchKernelGGL((eig_global_kernel<T>), dim3(global_grid(L)), dim3(GL_THREADS), 0, s,
+                     L, nm, n, a, w, info, solver, zws);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s, int solver, int *info)
 {
   if (bad_args(L, nm, n, m)) return hipErrorInvalidValue;
@@ -50,7 +71,8 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, solver, s);
   if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
   if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
-  if (n <= 128) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
+  if (use_wave_kernel(n, (int)sizeof(T))) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
+  if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
   return hipErrorNotSupported;
 }
 

@@ -87,7 +87,21 @@ def test_sym_rand_vs_oracle(n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [5, 32, 64, 100])
+@pytest.mark.parametrize("n", [129, 160, 255, 256, 300, 512])
+def test_large_n_vs_oracle(n, dtype):
+    print(f"[parity] large sym_rand n={n} dtype={np.dtype(dtype).name}", flush=True)
+    nm = n + (n & 1)
+    L = 5 if n <= 256 else 2
+    a0 = wl.sym_rand(L, n, nm, dtype=dtype, seed=3000 + n)
+    zo, wo, info_o = oracle.eig_batch(np.nan_to_num(a0), oracle.IMTQL2)
+    assert info_o.max() == 0
+    z, w, info = run_gpu(a0)
+    assert info.max() == 0
+    check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [5, 32, 64, 100, 200])
 def test_tql2_variant(n, dtype):
     print(f"[parity] tql2 n={n} dtype={np.dtype(dtype).name}", flush=True)
     a0 = wl.sym_rand(19, n, n, dtype=dtype, seed=77 + n)
@@ -98,7 +112,7 @@ def test_tql2_variant(n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [3, 10, 32, 48, 64, 97, 128])
+@pytest.mark.parametrize("n", [3, 10, 32, 48, 64, 97, 128, 256])
 def test_frank_closed_form(n, dtype):
     print(f"[parity] frank n={n} dtype={np.dtype(dtype).name}", flush=True)
     nm = n + (n & 1)
