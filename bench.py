@@ -50,6 +50,59 @@ def make_letkf_gpu(L, k, p, rho, sigma, seed, dtype):
     return A.to(dtype).contiguous()
 
 
+def make_qdq_gpu(L, n, seed, dtype, frac=0.05, gap=1e-8):
+    """A = Q diag(lam) Q^T, Q from the QR of a Gaussian matrix, lam ~ U[-1,1]
+    with a fraction `frac` of the eigenvalues replaced by pairs split by `gap`."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    A = torch.empty((L, n, n), dtype=torch.float64, device="cuda")
+    npairs = max(1, int(round(frac * n / 2)))
+    chunk = 64                      # batched QR workspace grows with the batch
+    for s in range(0, L, chunk):
+        e = min(L, s + chunk)
+        G = torch.randn((e - s, n, n), generator=g, device="cuda", dtype=torch.float64)
+        Q, _ = torch.linalg.qr(G)
+        lam = torch.rand((e - s, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        idx = torch.argsort(torch.rand((e - s, n), generator=g, device="cuda"), dim=1)[:, : 2 * npairs]
+        src = torch.gather(lam, 1, idx[:, 0::2])
+        lam.scatter_(1, idx[:, 1::2], src + gap)
+        QL = Q * lam[:, None, :]
+        A[s:e] = torch.bmm(QL, Q.transpose(1, 2).contiguous())
+    A = 0.5 * (A + A.transpose(1, 2))
+    return A.to(dtype).contiguous()
+
+
+def time_solver(a0, solver, steps, warmup):
+    """Mean HIP-event time (ms) of one eigen_GPU_batch call over the batch."""
+    import numpy as np
+    import torch
+    import eigeng_batched as eg
+    L, n, _ = a0.shape
+    a = torch.empty_like(a0)
+    w = torch.empty((L, n), dtype=a0.dtype, device="cuda")
+    ws = torch.empty(max(eg.bufsize(L, n, n, n, a.element_size(), solver), 1), dtype=torch.uint8,
+                     device="cuda")
+    info = torch.empty(L, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        a.copy_(a0)
+        eg.eigen_batch(a, w=w, solver=solver, info=info, workspace=ws, stream=stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    wall0 = time.perf_counter()
+    for i in range(steps):
+        a.copy_(a0)
+        ev[i][0].record(stream)
+        eg.eigen_batch(a, w=w, solver=solver, info=info, workspace=ws, stream=stream)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    return ms, wall, a, w, info
+
+
 def cpu_baseline(a_host, min_seconds=10.0, max_reps=20):
     import numpy as np
     import oracle
@@ -94,6 +147,7 @@ def main():
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--solver", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-n256", action="store_true", help="skip the n=256 FP64 secondary line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
 
@@ -111,34 +165,7 @@ def main():
     L, n = args.L, args.n
     k_mem, p_obs, rho, sigma = n, 400, 1.1, 0.1
     a0 = make_letkf_gpu(L, k_mem, p_obs, rho, sigma, seed=0xE16E0002, dtype=dt)
-    a = torch.empty_like(a0)
-    w = torch.empty((L, n), dtype=dt, device="cuda")
-    tb = a.element_size()
-    ws = torch.empty(max(eg.bufsize(L, n, n, n, tb, args.solver), 1), dtype=torch.uint8, device="cuda")
-    info = torch.empty(L, dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
-
-    def step():
-        eg.eigen_batch(a, w=w, solver=args.solver, info=info, workspace=ws, stream=stream)
-
-    for _ in range(args.warmup):
-        a.copy_(a0)
-        step()
-    torch.cuda.synchronize()
-
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    for i in range(args.steps):
-        a.copy_(a0)
-        ev[i][0].record(stream)
-        step()
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    times_ms = [s.elapsed_time(e) for s, e in ev]
-    ms = float(np.mean(times_ms))
+    ms, wall, a, w, info = time_solver(a0, args.solver, args.steps, args.warmup)
     mats = L / (ms * 1e-3)
 
     nbad = int((info != 0).sum().item())
@@ -186,6 +213,18 @@ def main():
         "nonconverged": nbad,
         "accuracy": acc,
     }
+    if not args.no_n256:
+        L2, n2 = 8192, 256
+        q0 = make_qdq_gpu(L2, n2, seed=0xE16E0004, dtype=torch.float64)
+        ms2, _, a2, w2, info2 = time_solver(q0, args.solver, 2, 1)
+        acc2 = accuracy_sample(q0.cpu().numpy(), a2.cpu().numpy(), w2.cpu().numpy(), count=32)
+        del q0, a2
+        out["n256"] = {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs "
+                                   "split by 1e-8", "value": L2 / (ms2 * 1e-3), "unit": "matrices/s",
+                       "ms_per_step": ms2, "nonconverged": int((info2 != 0).sum().item()),
+                       "accuracy": acc2,
+                       "roofline_fp64": {"achieved": L2 / (ms2 * 1e-3) * (10.0 / 3.0) * n2 ** 3 / 1e12,
+                                         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}}
     if not args.no_cpu:
         sample = a0_host[: min(L, 16384)]
         v, cores, reps, el, cbad = cpu_baseline(sample, args.cpu_seconds)
