@@ -1,0 +1,279 @@
+// eig_split.hpp - n <= 64 solver split into three kernels so that the scalar QL
+// recurrence runs one matrix per lane instead of one matrix per wave.
+//
+//   eig_tri_kernel    (group of NB lanes per matrix): load, Householder
+//                     tridiagonalisation, Q = H_{n-1}...H_1 (wave_front), then
+//                     Q -> workspace (n x n) and d, e -> workspace in a
+//                     matrix-minor layout (d_i of matrix m at de[i*ld + m]).
+//   eig_ql_lane_kernel (one matrix per lane, 64 per wave): IMTQL2 / TQL2 on the
+//                     lane's d, e (LDS, element i of lane l at [i][l], bank
+//                     conflict free), every Givens pair appended to the
+//                     matrix's rotation log, every sweep to its sweep log as
+//                     (l, m, lowest pair); then the reference's selection sort
+//                     (tql2_tiled.hpp:149-176) gives w and the output slot of
+//                     each column.
+//   eig_apply_kernel  (group of NB lanes per matrix): Q back into registers,
+//                     the logged sweeps replayed as rotation chains (lane-local,
+//                     next sweep's log prefetched during the current chain),
+//                     permuted store into a.
+// A matrix whose log would exceed its capacity is flagged and solved again by
+// the fused kernel (eig_wave_kernel with the `only` filter); its input is still
+// untouched at that point because nothing above writes a.
+#pragma once
+#include "common.hpp"
+#include "eig_wave.hpp"
+
+namespace eigmi {
+
+// rotation log capacity per matrix (pairs) and sweep log capacity; the
+// measured maxima are 0.93 n^2 rotations / 1.7 n sweeps (FP32, n = 64) and
+// 1.25 n^2 / 2.3 n (FP64, n = 32) over random and LETKF matrices
+__host__ __device__ constexpr int rot_cap_of(int n) { return (3 * n * n) / 2 + 16; }
+__host__ __device__ constexpr int sw_cap_of(int n) { return 3 * n + 8; }
+
+// sweep code: l | m << 8 | lowest pair << 16  (n <= 128)
+__device__ __forceinline__ int sw_code(int l, int m, int ilo) { return l | (m << 8) | (ilo << 16); }
+
+template <typename T, int NB>
+__global__ __launch_bounds__(64) void eig_tri_kernel(int L, int nm, int n, const T *__restrict__ a,
+                                                     T *__restrict__ qws, T *__restrict__ de, int ld)
+{
+  static_assert(GroupShape<NB>::NW == 1, "split path is for n <= 64");
+  constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NW = 1;
+  __shared__ WaveLds<T, NB> smem[G];
+  const int lane = threadIdx.x & 63;
+  const int g = lane / NL, kl = lane % NL, k = kl;
+  const long long mat = (long long)blockIdx.x * G + g;
+  if (mat >= L) return;
+  WaveLds<T, NB> &S = smem[g];
+  const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
+  T R[NB], dv[NW], ev[NW];
+  wave_front<T, NB>(A, nm, n, k, kl, 0, S, R, dv, ev);
+  T *__restrict__ Q = qws + (size_t)mat * (size_t)n * (size_t)n;
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    if (j < n && k < n) Q[k + (size_t)j * n] = R[j];
+  if (k < n) {
+    de[(size_t)k * ld + mat] = dv[0];
+    de[(size_t)(n + k) * ld + mat] = ev[0];
+  }
+}
+
+template <typename T, int NMAX>
+__global__ __launch_bounds__(64) void eig_ql_lane_kernel(
+    int L, int n, const T *__restrict__ de, int ld, typename vec2<T>::type *__restrict__ rotlog,
+    int rot_cap, int *__restrict__ swlog, int sw_cap, int *__restrict__ nsw_out, int *__restrict__ status,
+    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int solver)
+{
+  typedef typename vec2<T>::type T2;
+  __shared__ T dS[NMAX][64];
+  __shared__ T eS[NMAX][64];
+  const int lane = threadIdx.x & 63;
+  const long long mat = (long long)blockIdx.x * 64 + lane;
+  if (mat >= L) return;
+  for (int i = 0; i < n; ++i) {
+    dS[i][lane] = de[(size_t)i * ld + mat];
+    eS[i][lane] = de[(size_t)(n + i) * ld + mat];
+  }
+  T2 *__restrict__ lg = rotlog + (size_t)mat * rot_cap;
+  int *__restrict__ sw = swlog + (size_t)mat * sw_cap;
+  const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
+  int nrot = 0, nsw = 0, st = 0;
+
+  if (solver == SOLVER_TQL2) {
+    // explicit shift (tql2_tiled.hpp:40-146)
+    T shift = T(0), tst1 = T(0);
+    for (int l = 0; l < n && st == 0; ++l) {
+      const T h = (abs_(dS[l][lane]) + abs_(eS[l][lane])) * tol;
+      tst1 = (h > tst1) ? h : tst1;
+      int m = l;
+      while (m < n - 1 && abs_(eS[m][lane]) > tst1) ++m;
+      if (m != l) {
+        int itr;
+        for (itr = 0; itr < QL_MAX_SWEEP; ++itr) {
+          if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; break; }
+          const T dl_old = dS[l][lane], el = eS[l][lane];
+          const T p0 = (dS[l + 1][lane] - dl_old) / (el + el);
+          const T r0 = hypot1_c(p0);
+          const T psr = p0 + sign_(r0, p0);
+          const T dl = el / psr, dl1 = el * psr;
+          const T delta = dl_old - dl;
+          dS[l][lane] = dl;
+          dS[l + 1][lane] = dl1;
+          for (int i = l + 2; i < n; ++i) dS[i][lane] -= delta;
+          shift += delta;
+          T c = T(1), c2 = T(1), c3 = T(1), s = T(0), s2 = T(0);
+          T pp = dS[m][lane];
+          const T el1 = eS[l + 1][lane];
+          for (int i = m - 1; i >= l; --i) {
+            c3 = c2; c2 = c; s2 = s;
+            const T ei = eS[i][lane], di = dS[i][lane];
+            const T gg = c * ei, hh = c * pp;
+            T r, cn, sn;
+            if (!givens(ei, pp, r, cn, sn)) { cn = T(1); sn = T(0); }
+            eS[i + 1][lane] = s * r;
+            s = sn;
+            c = cn;
+            pp = c * di - s * gg;
+            dS[i + 1][lane] = hh + s * (c * gg + s * di);
+            lg[nrot++] = T2{c, s};
+          }
+          const T rr = (-s * s2 * c3 * el1 * el) / dl1;
+          eS[l][lane] = s * rr;
+          dS[l][lane] = c * rr;
+          sw[nsw++] = sw_code(l, m, l);
+          if (abs_(eS[l][lane]) <= tst1) break;
+        }
+        if (st == 0 && itr >= QL_MAX_SWEEP) st = l + 1;
+      }
+      if (st == 0) dS[l][lane] += shift;
+    }
+  } else {
+    // implicit shift (imtql2_tiled.hpp:235-321)
+    for (int l = 0; l < n && st == 0; ++l) {
+      for (int itr = 0;; ++itr) {
+        if (itr > QL_MAX_SWEEP) { st = l + 1; break; }
+        int m = l;
+        while (m < n - 1 && abs_(eS[m][lane]) > (abs_(dS[m][lane]) + abs_(dS[m + 1][lane])) * tol) ++m;
+        if (m == l) break;
+        if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; break; }
+        const T dl = dS[l][lane], el = eS[l][lane];
+        const T f0 = (dS[l + 1][lane] - dl) / (el + el);
+        const T g0 = hypot1_c(f0);
+        T di1 = dS[m][lane];
+        T r = (di1 - dl) + el / (f0 + sign_(g0, f0));
+        T s = T(1), c = T(1), delta = T(0);
+        T ei = eS[m - 1][lane], di = dS[m - 1][lane];
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          const int ip = (i > 0) ? i - 1 : 0;
+          const T ein = eS[ip][lane], din = dS[ip][lane];
+          const T f = s * ei, gg = r;
+          const T bb = c * ei;
+          if (!givens(f, gg, r, c, s)) break;
+          const T dix = di1 - delta;
+          di1 = di;
+          const T q = (di1 - dix) * s + T(2) * c * bb;
+          const T dx1 = fma_(s, q, dix);
+          delta = dx1 - dix;
+          eS[i + 1][lane] = r;
+          dS[i + 1][lane] = dx1;
+          lg[nrot++] = T2{c, s};
+          r = c * q - bb;
+          ei = ein;
+          di = din;
+        }
+        const int j = (i + 1 > l) ? i + 1 : l;
+        dS[j][lane] -= delta;
+        eS[j][lane] = r;
+        eS[m][lane] = T(0);
+        sw[nsw++] = sw_code(l, m, (i + 1 > l) ? i + 1 : l);
+      }
+    }
+  }
+
+  status[mat] = st;
+  nsw_out[mat] = nsw;
+  if (st == ST_OVERFLOW) return;                 // solved again by the fused kernel
+  if (info) info[mat] = st;
+  if (st != 0) {
+    for (int i = 0; i < n; ++i) w[(size_t)mat * n + i] = dS[i][lane];
+    return;
+  }
+  // selection sort with the reference's rule; eS now holds the positions
+  for (int i = 0; i < n; ++i) eS[i][lane] = T(i);
+  for (int l = 0; l < n - 1; ++l) {
+    T dl = dS[l][lane];
+    int il = l;
+    for (int j = l + 1; j < n; ++j) {
+      const T dj = dS[j][lane];
+      if (dl > dj) { dl = dj; il = j; }
+    }
+    if (il != l) {
+      const T pl = eS[l][lane];
+      eS[l][lane] = eS[il][lane];
+      eS[il][lane] = pl;
+      dS[il][lane] = dS[l][lane];
+      dS[l][lane] = dl;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    w[(size_t)mat * n + i] = dS[i][lane];
+    slot[(size_t)mat * n + (int)eS[i][lane]] = i;
+  }
+}
+
+template <typename T, int NB>
+struct alignas(16) ApplyLds {
+  alignas(16) typename vec2<T>::type cs[NB];
+  alignas(16) int pos[NB];
+};
+
+template <typename T, int NB>
+__global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *__restrict__ a,
+                                                       const T *__restrict__ qws,
+                                                       const typename vec2<T>::type *__restrict__ rotlog,
+                                                       int rot_cap, const int *__restrict__ swlog, int sw_cap,
+                                                       const int *__restrict__ nsw_in,
+                                                       const int *__restrict__ status, const int *__restrict__ slot)
+{
+  typedef typename vec2<T>::type T2;
+  constexpr int G = GroupShape<NB>::G;
+  static_assert(GroupShape<NB>::NW == 1, "split path is for n <= 64");
+  __shared__ ApplyLds<T, NB> smem[G];
+  const int lane = threadIdx.x & 63;
+  const int g = lane / NB, k = lane % NB;
+  const long long mat = (long long)blockIdx.x * G + g;
+  if (mat >= L) return;
+  if (status[mat] != 0) return;
+  ApplyLds<T, NB> &S = smem[g];
+  const T2 *__restrict__ lg = rotlog + (size_t)mat * rot_cap;
+  const int *__restrict__ sw = swlog + (size_t)mat * sw_cap;
+  const T *__restrict__ Q = qws + (size_t)mat * (size_t)n * (size_t)n;
+
+  T R[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) R[j] = (j < n && k < n) ? Q[k + (size_t)j * n] : T(0);
+  S.cs[k] = T2{T(1), T(0)};
+  const int nsw = nsw_in[mat];
+  // software pipeline: the next sweep's header and this lane's log entry are
+  // loaded while the current sweep's chain runs
+  int code = (nsw > 0) ? sw[0] : 0;
+  int base = 0;
+  T2 q = T2{T(1), T(0)};
+  {
+    const int m = (code >> 8) & 255, ilo = code >> 16;
+    if (nsw > 0 && k < m - ilo) q = lg[k];
+  }
+  for (int s = 0; s < nsw; ++s) {
+    const int l = code & 255, m = (code >> 8) & 255, ilo = code >> 16;
+    (void)l;
+    const int cnt = m - ilo;
+    if (k < cnt) S.cs[m - 1 - k] = q;
+    base += cnt;
+    // prefetch sweep s+1
+    const int code2 = (s + 1 < nsw) ? sw[s + 1] : 0;
+    {
+      const int m2 = (code2 >> 8) & 255, ilo2 = code2 >> 16;
+      if (s + 1 < nsw && k < m2 - ilo2) q = lg[base + k];
+    }
+    wsync();
+    apply_sweep_rotations<T, NB>(R, S.cs, ilo, m - 1);
+    wsync();
+    if (k >= ilo && k <= m - 1) S.cs[k] = T2{T(1), T(0)};
+    code = code2;
+    wsync();
+  }
+  if (k < n) S.pos[k] = slot[(size_t)mat * n + k];
+  wsync();
+  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (j < n) {
+      const int r = S.pos[j];
+      if (k < n) A[k + (size_t)r * nm] = R[j];
+    }
+  }
+}
+
+} // namespace eigmi

@@ -32,6 +32,7 @@ namespace eigmi {
 
 enum { SOLVER_IMTQL2 = 0, SOLVER_TQL2 = 1, SOLVER_JACOBI = 2 };
 constexpr int QL_MAX_SWEEP = 100; // tql2_tiled.hpp:7
+constexpr int ST_OVERFLOW = -2;   // split path: rotation log full, solve with the fused kernel
 
 // Offset of packed Householder column c (1 <= c) when column c' holds
 // roundup8(c') entries: sum_{c'=1}^{c-1} 8*ceil(c'/8).
@@ -117,26 +118,18 @@ __device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const typenam
   }
 }
 
+// Front half shared by the fused and the split solvers: load row k of matrix A
+// (upper triangle only), Householder tridiagonalisation, Q = H_{n-1}...H_1 in R.
+// On return R holds row k of Q and dv / ev the lane-distributed diagonal and
+// off-diagonal (replicated in every wave of the group).
 template <typename T, int NB>
-__global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
-    int L, int nm, int n, T *__restrict__ a, T *__restrict__ w, int *__restrict__ info, int solver)
+__device__ __forceinline__ void wave_front(const T *__restrict__ A, int nm, int n, int k, int kl, int wv,
+                                           WaveLds<T, NB> &S, T (&R)[NB],
+                                           T (&dv)[GroupShape<NB>::NW], T (&ev)[GroupShape<NB>::NW])
 {
-  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW, G = GroupShape<NB>::G;
+  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW;
   constexpr int NBLK = NB / 8;
-  typedef typename vec2<T>::type T2;
-  __shared__ WaveLds<T, NB> smem[G];
-
-  const int lane = threadIdx.x & 63;
-  const int wv = (int)(threadIdx.x >> 6);   // wave of the group (NW > 1)
-  const int g = lane / NL, kl = lane % NL;
-  const int k = wv * NL + kl;               // matrix row owned by this lane
-  const long long mat = (long long)blockIdx.x * G + g;
-  if (mat >= L) return;  // whole group leaves together (only for NW == 1; otherwise G == 1)
-  WaveLds<T, NB> &S = smem[g];
-  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
-
-  // R holds row k of A during the reduction and row k of Z afterwards.
-  T R[NB];
+  // R holds row k of A during the reduction and row k of Q afterwards.
   // ---- load: row k of the symmetric matrix from the upper triangle ----------
   // Columns j > k come from row k of the upper triangle (coalesced across
   // lanes); columns j <= k come from column k (a(j,k), j <= k, is the upper
@@ -168,7 +161,7 @@ __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
     }
   }
 
-  T dv[NW], ev[NW], bv[NW];
+  T bv[NW];
 #pragma unroll
   for (int h = 0; h < NW; ++h) { dv[h] = T(0); ev[h] = T(0); bv[h] = T(0); }
 
@@ -260,6 +253,31 @@ __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
       }
     }
   }
+
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
+    int L, int nm, int n, T *__restrict__ a, T *__restrict__ w, int *__restrict__ info, int solver,
+    const int *__restrict__ only)
+{
+  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW, G = GroupShape<NB>::G;
+  constexpr int NBLK = NB / 8;
+  typedef typename vec2<T>::type T2;
+  __shared__ WaveLds<T, NB> smem[G];
+
+  const int lane = threadIdx.x & 63;
+  const int wv = (int)(threadIdx.x >> 6);   // wave of the group (NW > 1)
+  const int g = lane / NL, kl = lane % NL;
+  const int k = wv * NL + kl;               // matrix row owned by this lane
+  const long long mat = (long long)blockIdx.x * G + g;
+  if (mat >= L) return;  // whole group leaves together (only for NW == 1; otherwise G == 1)
+  if (only && only[mat] != ST_OVERFLOW) return;  // fallback pass of the split path
+  WaveLds<T, NB> &S = smem[g];
+  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
+
+  T R[NB], dv[NW], ev[NW];
+  wave_front<T, NB>(A, nm, n, k, kl, wv, S, R, dv, ev);
 
   // ---- QL with eigenvectors accumulated into Z = R --------------------------
   // Everything below is per wave (replicated vectors, a private (c, s) array):

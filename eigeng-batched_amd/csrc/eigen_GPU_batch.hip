@@ -13,6 +13,8 @@
 #include "../../include/eigen_GPU_batch.h"
 #include "eig_wave.hpp"
 #include "eig_global.hpp"
+#include "eig_jacobi.hpp"
+#include "eig_split.hpp"
 
 using namespace eigmi;
 
@@ -21,6 +23,39 @@ namespace {
 bool bad_args(int L, int nm, int n, int m) { return L < 1 || nm < n || n < 1 || m < 1 || m > n; }
 
 constexpr int WS_HEAD = 256;      // bytes; keeps the workspace non-empty and aligned
+constexpr int LCHUNK = 16384;     // split path: matrices per chunk (bounds the workspace)
+constexpr int SPLIT_MIN_L = 2048; // split path used from this batch size on (n <= 64)
+// solver tuning flags (_x entry points): force the fused or the split n <= 64
+// path; FLAG_TINYLOG shrinks the split path's rotation log so that every
+// matrix takes the overflow fallback (a host-forced test of that path)
+constexpr int FLAG_FUSED = 0x100, FLAG_SPLIT = 0x200, FLAG_TINYLOG = 0x400;
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Split-path workspace regions for a chunk of lc matrices.
+struct SplitLayout {
+  size_t q, de, rot, sw, nsw, status, slot, total;
+  SplitLayout(int lc, int n, int tb)
+  {
+    q = 0;
+    de = q + al256((size_t)lc * n * n * tb);
+    rot = de + al256((size_t)2 * n * lc * tb);
+    sw = rot + al256((size_t)lc * rot_cap_of(n) * 2 * tb);
+    nsw = sw + al256((size_t)lc * sw_cap_of(n) * 4);
+    status = nsw + al256((size_t)lc * 4);
+    slot = status + al256((size_t)lc * 4);
+    total = slot + al256((size_t)lc * n * 4);
+  }
+};
+
+bool use_split(int L, int n, int solver)
+{
+  const int base = solver & 0xff, flags = solver & ~0xff;
+  if (n > 64 || (base != SOLVER_IMTQL2 && base != SOLVER_TQL2)) return false;
+  if (flags & FLAG_FUSED) return false;
+  if (flags & FLAG_SPLIT) return true;
+  return L >= SPLIT_MIN_L;
+}
 
 // Which kernel serves (n, element size): register-resident groups while a row
 // of the matrix fits in the register file, the workgroup/global-memory kernel
@@ -35,7 +70,8 @@ int global_grid(int L) { return L < 1024 ? L : 1024; }
 // accumulator per resident workgroup.
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
-  (void)nm; (void)m; (void)solver;
+  (void)nm; (void)m;
+  if (use_split(L, n, solver)) return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
   if (use_wave_kernel(n, tbytes)) return WS_HEAD;
   if (n > GL_NMAX) return 0;
   return WS_HEAD + (size_t)global_grid(L) * (size_t)n * (size_t)n * (size_t)tbytes;
@@ -48,7 +84,47 @@ hipError_t launch_wave(int L, int nm, int n, T *a, T *w, int *info, int solver, 
   const long long blocks = ((long long)L + G - 1) / G;
   if (blocks > 0x7fffffffll) return hipErrorInvalidValue;
   hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3((unsigned)blocks), dim3(64 * NW), 0, s,
-                     L, nm, n, a, w, info, solver);
+                     L, nm, n, a, w, info, solver & 0xff, (const int *)nullptr);
+  return hipGetLastError();
+}
+
+template <typename T, int NB>
+hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
+{
+  typedef typename vec2<T>::type T2;
+  constexpr int G = GroupShape<NB>::G;
+  const int lc = L < LCHUNK ? L : LCHUNK;
+  const SplitLayout lay(lc, n, (int)sizeof(T));
+  char *base = (char *)wk + WS_HEAD;
+  T *q = (T *)(base + lay.q), *de = (T *)(base + lay.de);
+  T2 *rot = (T2 *)(base + lay.rot);
+  int *sw = (int *)(base + lay.sw), *nsw = (int *)(base + lay.nsw);
+  int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
+  for (long long c0 = 0; c0 < L; c0 += lc) {
+    const int cur = (int)((L - c0) < lc ? (L - c0) : lc);
+    T *ac = a + (size_t)c0 * nm * n;
+    T *wc = w + (size_t)c0 * n;
+    int *ic = info ? info + c0 : nullptr;
+    const unsigned gb = (unsigned)((cur + G - 1) / G), lb = (unsigned)((cur + 63) / 64);
+    hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, de, lc);
+    const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
+    hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap, sw,
+                       scap, nsw, st, slot, wc, ic, solver & 0xff);
+    hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
+                       sw, scap, nsw, st, slot);
+    hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, wc, ic,
+                       solver & 0xff, (const int *)st);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int NB>
+hipError_t launch_jacobi(int L, int nm, int n, T *a, T *w, int *info, hipStream_t s)
+{
+  constexpr int G = 64 / NB;
+  const long long blocks = ((long long)L + G - 1) / G;
+  if (blocks > 0x7fffffffll) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((eig_jacobi_kernel<T, NB>), dim3((unsigned)blocks), dim3(64), 0, s, L, nm, n, a, w, info);
   return hipGetLastError();
 }
 
@@ -65,14 +141,28 @@ template <typename T>
 hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s, int solver, int *info)
 {
   if (bad_args(L, nm, n, m)) return hipErrorInvalidValue;
-  if (solver != SOLVER_IMTQL2 && solver != SOLVER_TQL2) return hipErrorNotSupported;
+  const int base = solver & 0xff;
+  if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2 && base != SOLVER_JACOBI) return hipErrorNotSupported;
   if (wk == nullptr) return hipErrorInvalidValue;
+  if (use_split(L, n, solver)) {
+    if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, wk, s);
+    if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, wk, s);
+    if (n <= 32) return launch_split<T, 32>(L, nm, n, a, w, info, solver, wk, s);
+    return launch_split<T, 64>(L, nm, n, a, w, info, solver, wk, s);
+  }
+  if (base == SOLVER_JACOBI) {
+    if (n <= 8) return launch_jacobi<T, 8>(L, nm, n, a, w, info, s);
+    if (n <= 16) return launch_jacobi<T, 16>(L, nm, n, a, w, info, s);
+    if (n <= 32) return launch_jacobi<T, 32>(L, nm, n, a, w, info, s);
+    if (n <= 64) return launch_jacobi<T, 64>(L, nm, n, a, w, info, s);
+    return hipErrorNotSupported;
+  }
   if (n <= 8) return launch_wave<T, 8>(L, nm, n, a, w, info, solver, s);
   if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, solver, s);
   if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
   if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
   if (use_wave_kernel(n, (int)sizeof(T))) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
-  if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
+  if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, base, wk, s);
   return hipErrorNotSupported;
 }
 

@@ -20,6 +20,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeigenGbatch.so")
 
 IMTQL2, TQL2, JACOBI = 0, 1, 2
+# tuning flags OR-ed into the solver id of the _x entry points
+FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
+PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG}
 _lib = None
 
 
@@ -70,13 +73,16 @@ def _stream_ptr(stream) -> int:
 
 
 def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None,
-                workspace=None, stream=None, abi: bool = False):
+                workspace=None, stream=None, abi: bool = False, path: str = "auto"):
     """Solve in place: ``a`` (L, n, nm) float32/float64 CUDA tensor.
 
     Returns (a, w, info): eigenvectors overwrite ``a`` (column j of matrix k is
     ``a[k, j, :n]``), ``w`` (L, n) ascending eigenvalues, ``info`` (L,) int32
     per-matrix status (0 ok, l>0 QL did not converge) or None when abi=True.
     abi=True calls the reference-compatible eigen_GPU_batch_FP/_DP symbol.
+    path selects the n <= 64 implementation: "auto", "fused" (one kernel),
+    "split" (lane-per-matrix QL recurrence) or "split-fallback" (split with the
+    rotation log shrunk so that every matrix takes the overflow fallback).
     """
     import torch
     if not a.is_cuda:
@@ -86,6 +92,7 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     L, n, nm = a.shape
     m = n if m is None else m
     tb = a.element_size()
+    solver = solver | PATHS[path]
     if w is None:
         w = torch.empty((L, n), dtype=a.dtype, device=a.device)
     if workspace is None:

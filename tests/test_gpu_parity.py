@@ -26,9 +26,9 @@ def ref_tol(n, dtype):
     return eps * (1023.0 if dtype == np.float64 else 31.0) * np.sqrt(n + 16.0)
 
 
-def run_gpu(a_np, solver=eg.IMTQL2, abi=False):
+def run_gpu(a_np, solver=eg.IMTQL2, abi=False, path="auto"):
     t = torch.from_numpy(np.ascontiguousarray(a_np)).to(DEV)
-    t, w, info = eg.eigen_batch(t, solver=solver, abi=abi)
+    t, w, info = eg.eigen_batch(t, solver=solver, abi=abi, path=path)
     torch.cuda.synchronize()
     return (t.cpu().numpy(), w.cpu().numpy(),
             None if info is None else info.cpu().numpy())
@@ -86,6 +86,37 @@ def test_sym_rand_vs_oracle(n, dtype):
     check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
 
 
+SPLIT_SIZES = [1, 2, 3, 5, 8, 9, 16, 17, 31, 32, 33, 48, 63, 64]
+
+
+@pytest.mark.parametrize("path", ["split", "split-fallback"])
+@pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", SPLIT_SIZES)
+def test_split_path_vs_oracle(n, dtype, solver, path):
+    print(f"[parity] {path} solver={solver} n={n} dtype={np.dtype(dtype).name}", flush=True)
+    nm = n + (n & 1)
+    L = 70
+    a0 = wl.sym_rand(L, n, nm, dtype=dtype, seed=5000 + n)
+    zo, wo, info_o = oracle.eig_batch(np.nan_to_num(a0), oracle.TQL2 if solver == eg.TQL2 else oracle.IMTQL2)
+    assert info_o.max() == 0
+    z, w, info = run_gpu(a0, solver=solver, path=path)
+    assert info.max() == 0 and info.min() == 0
+    if nm > n:
+        assert np.isnan(z[:, :, n:]).all()
+    check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_split_and_fused_agree_on_letkf(dtype):
+    a0, _ = wl.letkf_y(300, k=48, p=300, rho=1.05, sigma=0.1, dtype=dtype, seed=17)
+    z1, w1, _ = run_gpu(a0, path="fused")
+    z2, w2, _ = run_gpu(a0, path="split")
+    tol = ref_tol(48, dtype)
+    assert np.max(np.abs(w1 - w2) / np.abs(w1).max(axis=1, keepdims=True)) <= tol
+    check_batch(a0, z2, w2, dtype, w1, z1)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [129, 160, 255, 256, 300, 512])
 def test_large_n_vs_oracle(n, dtype):
@@ -109,6 +140,34 @@ def test_tql2_variant(n, dtype):
     z, w, info = run_gpu(a0, solver=eg.TQL2)
     assert info.max() == 0
     check_batch(a0, z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 8, 9, 16, 20, 32, 33, 48, 64])
+def test_jacobi_vs_oracle(n, dtype):
+    print(f"[parity] jacobi n={n} dtype={np.dtype(dtype).name}", flush=True)
+    nm = n + (n & 1)
+    a0 = wl.sym_rand(21, n, nm, dtype=dtype, seed=4000 + n)
+    zo, wo, _ = oracle.eig_batch(np.nan_to_num(a0), oracle.JACOBI)
+    z, w, info = run_gpu(a0, solver=eg.JACOBI)
+    assert info.min() == 0 and info.max() == 0
+    if nm > n:
+        assert np.isnan(z[:, :, n:]).all()
+    check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_jacobi_closed_forms(dtype):
+    n = 40
+    a0, lam = wl.letkf_helmert(6, n, n, dtype=dtype, seed=8)
+    z, w, info = run_gpu(a0, solver=eg.JACOBI)
+    assert info.max() == 0
+    assert np.max(np.abs(w - lam) / lam[:, -1:]) <= ref_tol(n, dtype)
+    check_batch(a0, z, w, dtype, vec_check=False)
+    a1 = wl.frank(3, 30, 30, dtype=dtype)
+    z, w, info = run_gpu(a1, solver=eg.JACOBI)
+    assert np.max(np.abs(w - wl.frank_eigenvalues(30)) / w[:, -1:]) <= ref_tol(30, dtype)
+    check_batch(a1, z, w, dtype)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
