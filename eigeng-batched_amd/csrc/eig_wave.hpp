@@ -256,26 +256,15 @@ __device__ __forceinline__ void wave_front(const T *__restrict__ A, int nm, int 
 
 }
 
+// Whole fused solve of one matrix by its group: A (leading dimension nm, order
+// n), eigenvalues to wout[0..n-1], status to *infop (may be null).
 template <typename T, int NB>
-__global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
-    int L, int nm, int n, T *__restrict__ a, T *__restrict__ w, int *__restrict__ info, int solver,
-    const int *__restrict__ only)
+__device__ __forceinline__ void wave_solve(T *__restrict__ A, int nm, int n, T *__restrict__ wout,
+                                           int *__restrict__ infop, int solver, WaveLds<T, NB> &S,
+                                           int lane, int wv, int kl, int k)
 {
-  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW, G = GroupShape<NB>::G;
-  constexpr int NBLK = NB / 8;
+  constexpr int NL = GroupShape<NB>::NL, NW = GroupShape<NB>::NW;
   typedef typename vec2<T>::type T2;
-  __shared__ WaveLds<T, NB> smem[G];
-
-  const int lane = threadIdx.x & 63;
-  const int wv = (int)(threadIdx.x >> 6);   // wave of the group (NW > 1)
-  const int g = lane / NL, kl = lane % NL;
-  const int k = wv * NL + kl;               // matrix row owned by this lane
-  const long long mat = (long long)blockIdx.x * G + g;
-  if (mat >= L) return;  // whole group leaves together (only for NW == 1; otherwise G == 1)
-  if (only && only[mat] != ST_OVERFLOW) return;  // fallback pass of the split path
-  WaveLds<T, NB> &S = smem[g];
-  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
-
   T R[NB], dv[NW], ev[NW];
   wave_front<T, NB>(A, nm, n, k, kl, wv, S, R, dv, ev);
 
@@ -446,12 +435,12 @@ __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
 #pragma unroll
     for (int q = 0; q < NW; ++q) {
       const int idx = kl + q * NL;
-      if (q == wv && idx < n) w[(size_t)mat * n + idx] = dv[q];
+      if (q == wv && idx < n) wout[idx] = dv[q];
     }
-    if (info && k == 0) info[mat] = err;
+    if (infop && k == 0) *infop = err;
     return;
   }
-  if (info && k == 0) info[mat] = 0;
+  if (infop && k == 0) *infop = 0;
 
   // ---- ascending sort, the reference's selection rule (tql2_tiled.hpp:149-176):
   // step l swaps slot l with the first minimum of slots l..n-1.
@@ -494,7 +483,7 @@ __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
   for (int q = 0; q < NW; ++q) {
     const int idx = kl + q * NL;
     if (idx < n) {
-      if (q == wv) w[(size_t)mat * n + idx] = dv[q];
+      if (q == wv) wout[idx] = dv[q];
       posw[pv[q]] = idx;
     }
   }
@@ -506,6 +495,25 @@ __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
       if (k < n) A[k + (size_t)r * nm] = R[j];
     }
   }
+}
+
+
+template <typename T, int NB>
+__global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_wave_kernel(
+    int L, int nm, int n, T *__restrict__ a, T *__restrict__ w, int *__restrict__ info, int solver,
+    const int *__restrict__ only)
+{
+  constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G;
+  __shared__ WaveLds<T, NB> smem[G];
+  const int lane = threadIdx.x & 63;
+  const int wv = (int)(threadIdx.x >> 6);   // wave of the group (NW > 1)
+  const int g = lane / NL, kl = lane % NL;
+  const int k = wv * NL + kl;               // matrix row owned by this lane
+  const long long mat = (long long)blockIdx.x * G + g;
+  if (mat >= L) return;  // whole group leaves together (only for NW == 1; otherwise G == 1)
+  if (only && only[mat] != ST_OVERFLOW) return;  // fallback pass of the split path
+  wave_solve<T, NB>(a + (size_t)mat * (size_t)nm * (size_t)n, nm, n, w + (size_t)mat * n,
+                    info ? info + mat : nullptr, solver, smem[g], lane, wv, kl, k);
 }
 
 } // namespace eigmi

@@ -15,6 +15,11 @@
 #include "eig_global.hpp"
 #include "eig_jacobi.hpp"
 #include "eig_split.hpp"
+#include "eig_uneven.hpp"
+
+#include <mutex>
+#include <map>
+#include <utility>
 
 using namespace eigmi;
 
@@ -29,6 +34,12 @@ constexpr int SPLIT_MIN_L = 2048; // split path used from this batch size on (n 
 // path; FLAG_TINYLOG shrinks the split path's rotation log so that every
 // matrix takes the overflow fallback (a host-forced test of that path)
 constexpr int FLAG_FUSED = 0x100, FLAG_SPLIT = 0x200, FLAG_TINYLOG = 0x400;
+// split path phase selection: a call with some of these bits runs only those
+// phases (tridiagonalisation + Q, QL recurrence, rotation apply + fallback);
+// the phases hand over through the workspace, so consecutive calls with the
+// same workspace and stream compose the full solve (batches of at most LCHUNK)
+constexpr int FLAG_PH_TRI = 0x1000, FLAG_PH_QL = 0x2000, FLAG_PH_APPLY = 0x4000;
+constexpr int FLAG_PH_ALL = FLAG_PH_TRI | FLAG_PH_QL | FLAG_PH_APPLY;
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -100,20 +111,26 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
   T2 *rot = (T2 *)(base + lay.rot);
   int *sw = (int *)(base + lay.sw), *nsw = (int *)(base + lay.nsw);
   int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
+  const int ph = (solver & FLAG_PH_ALL) ? (solver & FLAG_PH_ALL) : FLAG_PH_ALL;
+  if (ph != FLAG_PH_ALL && L > lc) return hipErrorInvalidValue;
   for (long long c0 = 0; c0 < L; c0 += lc) {
     const int cur = (int)((L - c0) < lc ? (L - c0) : lc);
     T *ac = a + (size_t)c0 * nm * n;
     T *wc = w + (size_t)c0 * n;
     int *ic = info ? info + c0 : nullptr;
     const unsigned gb = (unsigned)((cur + G - 1) / G), lb = (unsigned)((cur + 63) / 64);
-    hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, de, lc);
     const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
-    hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap, sw,
-                       scap, nsw, st, slot, wc, ic, solver & 0xff);
-    hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
-                       sw, scap, nsw, st, slot);
-    hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, wc, ic,
-                       solver & 0xff, (const int *)st);
+    if (ph & FLAG_PH_TRI)
+      hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, de, lc);
+    if (ph & FLAG_PH_QL)
+      hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap, sw,
+                         scap, nsw, st, slot, wc, ic, solver & 0xff);
+    if (ph & FLAG_PH_APPLY) {
+      hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
+                         sw, scap, nsw, st, slot);
+      hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, wc, ic,
+                         solver & 0xff, (const int *)st);
+    }
   }
   return hipGetLastError();
 }
@@ -144,6 +161,7 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   const int base = solver & 0xff;
   if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2 && base != SOLVER_JACOBI) return hipErrorNotSupported;
   if (wk == nullptr) return hipErrorInvalidValue;
+  if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver)) return hipErrorInvalidValue;
   if (use_split(L, n, solver)) {
     if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, wk, s);
     if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, wk, s);
@@ -164,6 +182,106 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (use_wave_kernel(n, (int)sizeof(T))) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
   if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, base, wk, s);
   return hipErrorNotSupported;
+}
+
+// ---- uneven batches ---------------------------------------------------------
+
+// Size classes per element type: register groups of 8..64 lanes (and two-wave
+// groups of 128 rows for FP32), the workgroup kernel beyond.
+UvClasses uv_classes(int tbytes)
+{
+  UvClasses c{};
+  const int f32[] = {8, 16, 32, 64, 128, GL_NMAX};
+  const int f64[] = {8, 16, 32, 64, GL_NMAX};
+  const int *h = tbytes == 4 ? f32 : f64;
+  c.ncls = tbytes == 4 ? 6 : 5;
+  for (int i = 0; i < c.ncls; ++i) {
+    c.hi[i] = h[i];
+    c.g[i] = h[i] < 64 ? 64 / h[i] : 1;
+  }
+  return c;
+}
+
+int uv_global_grid(int L) { return L < 1024 ? L : 1024; }
+
+struct UvLayout {
+  size_t ctl, list, aoff, woff, z, total;
+  UvLayout(int L, int nmax, int tb)
+  {
+    ctl = 0;
+    list = ctl + al256(sizeof(int) * 4 * UV_MAXCLS);
+    aoff = list + al256((size_t)uv_list_len(L) * sizeof(int));
+    woff = aoff + al256((size_t)L * sizeof(long long));
+    z = woff + al256((size_t)L * sizeof(long long));
+    const bool glob = !use_wave_kernel(nmax, tb);
+    total = z + (glob ? al256((size_t)uv_global_grid(L) * nmax * nmax * tb) : 0);
+  }
+};
+
+// Workgroups that fit on the device at once for a kernel (cached per device
+// and kernel; the first call per pair queries the runtime).
+int resident_blocks(const void *kernel, int threads)
+{
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({dev, kernel});
+  if (it != cache.end()) return it->second;
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+  const int nb = cus * per;
+  cache[{dev, kernel}] = nb;
+  return nb;
+}
+
+template <typename T, int NB>
+void launch_wave_list(int *ctl, const int *list, const int *nv, const long long *ao, const long long *wo, T *a,
+                      T *w, int *info, int solver, hipStream_t s)
+{
+  constexpr int NW = GroupShape<NB>::NW;
+  const void *kern = (const void *)eig_wave_list_kernel<T, NB>;
+  const int blocks = resident_blocks(kern, 64 * NW);
+  hipLaunchKernelGGL((eig_wave_list_kernel<T, NB>), dim3(blocks), dim3(64 * NW), 0, s, ctl, list, nv, ao, wo,
+                     a, w, info, solver);
+}
+
+template <typename T>
+hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, const long long *woff, T *a, T *w,
+                      void *wk, hipStream_t s, int solver, int *info)
+{
+  if (L < 1 || nmax < 1 || nmax > GL_NMAX || !nv || !a || !w || !wk) return hipErrorInvalidValue;
+  if ((aoff == nullptr) != (woff == nullptr)) return hipErrorInvalidValue;
+  const int base = solver & 0xff;
+  if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2) return hipErrorNotSupported;
+  const int tb = (int)sizeof(T);
+  const UvLayout lay(L, nmax, tb);
+  char *ws = (char *)wk + WS_HEAD;
+  int *ctl = (int *)(ws + lay.ctl), *list = (int *)(ws + lay.list);
+  long long *aws = (long long *)(ws + lay.aoff), *wws = (long long *)(ws + lay.woff);
+  T *zws = (T *)(ws + lay.z);
+  const UvClasses cls = uv_classes(tb);
+  hipLaunchKernelGGL(uv_bin_kernel, dim3(1), dim3(UV_THREADS), 0, s, L, nmax, nv, aws, wws,
+                     aoff == nullptr ? 1 : 0, list, ctl, info, cls);
+  const long long *ao = aoff ? aoff : aws, *wo = woff ? woff : wws;
+  for (int c = 0; c < cls.ncls; ++c) {
+    const int lo = c == 0 ? 0 : cls.hi[c - 1];
+    if (lo >= nmax) break;
+    int *cc = ctl + 4 * c;
+    switch (cls.hi[c]) {
+    case 8: launch_wave_list<T, 8>(cc, list, nv, ao, wo, a, w, info, base, s); break;
+    case 16: launch_wave_list<T, 16>(cc, list, nv, ao, wo, a, w, info, base, s); break;
+    case 32: launch_wave_list<T, 32>(cc, list, nv, ao, wo, a, w, info, base, s); break;
+    case 64: launch_wave_list<T, 64>(cc, list, nv, ao, wo, a, w, info, base, s); break;
+    case 128: launch_wave_list<T, (sizeof(T) == 4 ? 128 : 64)>(cc, list, nv, ao, wo, a, w, info, base, s); break;
+    default:
+      hipLaunchKernelGGL((eig_global_list_kernel<T>), dim3(uv_global_grid(L)), dim3(GL_THREADS), 0, s, cc,
+                         list, nv, ao, wo, a, w, info, base, zws, (long long)nmax * nmax);
+    }
+  }
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -212,6 +330,25 @@ int eigen_GPU_batch_DP_x(int L, int nm, int n, int m, double *a, double *w, void
                          int solver, int *info)
 {
   return (int)run<double>(L, nm, n, m, a, w, wk, stream, solver, info);
+}
+
+size_t eigen_GPU_batch_uneven_bufsize(int L, int nmax, int dtype_bytes, int solver)
+{
+  (void)solver;
+  if (L < 1 || nmax < 1 || nmax > GL_NMAX || (dtype_bytes != 4 && dtype_bytes != 8)) return 0;
+  return WS_HEAD + UvLayout(L, nmax, dtype_bytes).total;
+}
+
+int eigen_GPU_batch_uneven_FP(int L, int nmax, const int *n, const long long *aoff, const long long *woff,
+                              float *a, float *w, void *wk, hipStream_t stream, int solver, int *info)
+{
+  return (int)run_uneven<float>(L, nmax, n, aoff, woff, a, w, wk, stream, solver, info);
+}
+
+int eigen_GPU_batch_uneven_DP(int L, int nmax, const int *n, const long long *aoff, const long long *woff,
+                              double *a, double *w, void *wk, hipStream_t stream, int solver, int *info)
+{
+  return (int)run_uneven<double>(L, nmax, n, aoff, woff, a, w, wk, stream, solver, info);
 }
 
 const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.1"; }

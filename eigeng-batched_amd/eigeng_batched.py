@@ -23,6 +23,10 @@ IMTQL2, TQL2, JACOBI = 0, 1, 2
 # tuning flags OR-ed into the solver id of the _x entry points
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
 PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG}
+# split-path phases (a call with a subset runs only those kernels; the phases
+# hand over through the workspace, so tri -> ql -> apply calls on one stream
+# and one workspace compose the full solve, for batches of at most 16384)
+PHASES = {"tri": 0x1000, "ql": 0x2000, "apply": 0x4000}
 _lib = None
 
 
@@ -47,6 +51,12 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, ci, vp]
             f.restype = ci
+        for name in ("eigen_GPU_batch_uneven_FP", "eigen_GPU_batch_uneven_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, vp]
+            f.restype = ci
+        L.eigen_GPU_batch_uneven_bufsize.argtypes = [ci, ci, ci, ci]
+        L.eigen_GPU_batch_uneven_bufsize.restype = sz
         L.eigen_GPU_batch_bufsize.argtypes = [ci, ci, ci, ci, ci, ci]
         L.eigen_GPU_batch_bufsize.restype = sz
         L.eigen_GPU_batch_version.restype = ctypes.c_char_p
@@ -73,7 +83,7 @@ def _stream_ptr(stream) -> int:
 
 
 def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None,
-                workspace=None, stream=None, abi: bool = False, path: str = "auto"):
+                workspace=None, stream=None, abi: bool = False, path: str = "auto", phases=None):
     """Solve in place: ``a`` (L, n, nm) float32/float64 CUDA tensor.
 
     Returns (a, w, info): eigenvectors overwrite ``a`` (column j of matrix k is
@@ -83,6 +93,7 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     path selects the n <= 64 implementation: "auto", "fused" (one kernel),
     "split" (lane-per-matrix QL recurrence) or "split-fallback" (split with the
     rotation log shrunk so that every matrix takes the overflow fallback).
+    phases (split path only): iterable of "tri", "ql", "apply" to run.
     """
     import torch
     if not a.is_cuda:
@@ -93,6 +104,9 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     m = n if m is None else m
     tb = a.element_size()
     solver = solver | PATHS[path]
+    if phases:
+        for ph in phases:
+            solver |= PHASES[ph]
     if w is None:
         w = torch.empty((L, n), dtype=a.dtype, device=a.device)
     if workspace is None:
@@ -110,6 +124,81 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
            info.data_ptr())
     if rc != 0:
         raise RuntimeError(f"eigen_GPU_batch_{'FP' if tb == 4 else 'DP'}_x failed: hipError {rc}")
+    return a, w, info
+
+
+UNEVEN_BAD_ORDER = -3  # info value of a matrix whose order is outside 1..nmax
+
+
+def uneven_bufsize(L: int, nmax: int, dtype_bytes: int, solver: int = IMTQL2) -> int:
+    return int(lib().eigen_GPU_batch_uneven_bufsize(L, nmax, dtype_bytes, solver))
+
+
+def pack_uneven(mats, device="cuda"):
+    """Pack square matrices (a list of (n_k, n_k) arrays/tensors, row r column c
+    of matrix k at mats[k][r, c]) into the packed uneven layout: returns
+    (a, nv) with a 1-D tensor holding matrix k column-major after matrix k-1
+    and nv the int32 orders."""
+    import torch
+    ns = [int(m.shape[0]) for m in mats]
+    parts = [torch.as_tensor(m).t().contiguous().reshape(-1) for m in mats]
+    a = torch.cat(parts).to(device)
+    nv = torch.tensor(ns, dtype=torch.int32, device=device)
+    return a, nv
+
+
+def unpack_uneven(a, w, ns):
+    """Inverse of pack_uneven for results: list of (Z_k, w_k) on the host, with
+    Z_k[:, j] the eigenvector of w_k[j]."""
+    a = a.cpu()
+    w = w.cpu()
+    out, oa, ow = [], 0, 0
+    for n in ns:
+        z = a[oa:oa + n * n].reshape(n, n).t().contiguous()
+        out.append((z.numpy(), w[ow:ow + n].numpy()))
+        oa += n * n
+        ow += n
+    return out
+
+
+def eigen_batch_uneven(a, nv, nmax: int, w=None, aoff=None, woff=None, solver: int = IMTQL2,
+                       info=None, workspace=None, stream=None):
+    """Solve L matrices of different orders in place (the uneven entry points).
+
+    ``a`` 1-D float32/float64 device tensor; ``nv`` (L,) int32 device tensor of
+    orders; ``nmax`` a host-side bound on the orders (<= 512).  Without
+    ``aoff``/``woff`` (int64 device tensors of element offsets) the layout is
+    packed and ``w`` defaults to a tensor of sum(n) elements (one host read of
+    nv).  Returns (a, w, info)."""
+    import torch
+    if not (a.is_cuda and nv.is_cuda):
+        raise ValueError("eigen_batch_uneven needs device tensors (no CPU path)")
+    if a.dtype not in (torch.float32, torch.float64) or a.dim() != 1 or not a.is_contiguous():
+        raise ValueError("a must be a contiguous 1-D float32/float64 tensor")
+    if nv.dtype != torch.int32 or nv.dim() != 1 or not nv.is_contiguous():
+        raise ValueError("nv must be a contiguous 1-D int32 tensor")
+    if (aoff is None) != (woff is None):
+        raise ValueError("give both aoff and woff, or neither (packed layout)")
+    for o in (aoff, woff):
+        if o is not None and (o.dtype != torch.int64 or not o.is_cuda or o.numel() != nv.numel()):
+            raise ValueError("aoff/woff must be (L,) int64 device tensors")
+    L = nv.numel()
+    tb = a.element_size()
+    if w is None:
+        if aoff is not None:
+            raise ValueError("w is required with explicit offsets")
+        w = torch.empty(int(nv.clamp(min=0).sum().item()), dtype=a.dtype, device=a.device)
+    if info is None:
+        info = torch.empty(L, dtype=torch.int32, device=a.device)
+    if workspace is None:
+        workspace = torch.empty(max(uneven_bufsize(L, nmax, tb, solver), 1), dtype=torch.uint8,
+                                device=a.device)
+    f = lib().eigen_GPU_batch_uneven_FP if tb == 4 else lib().eigen_GPU_batch_uneven_DP
+    rc = f(L, nmax, nv.data_ptr(), None if aoff is None else aoff.data_ptr(),
+           None if woff is None else woff.data_ptr(), a.data_ptr(), w.data_ptr(),
+           workspace.data_ptr(), _stream_ptr(stream), solver, info.data_ptr())
+    if rc != 0:
+        raise RuntimeError(f"eigen_GPU_batch_uneven_{'FP' if tb == 4 else 'DP'} failed: hipError {rc}")
     return a, w, info
 
 

@@ -55,6 +55,25 @@ enum eigen_GPU_batch_solver {
   EIGEN_GPU_BATCH_JACOBI = 2  /* two-sided Jacobi on the full matrix (jac_tiled.hpp) */
 };
 
+/*
+ * Tuning bits OR-ed into `solver` of the _x entry points and of
+ * eigen_GPU_batch_bufsize (the default picks by n and L):
+ *   0x100  one fused kernel per matrix group;
+ *   0x200  split n <= 64 path: register tridiagonalisation, one-lane-per-matrix
+ *          QL recurrence writing a rotation log, rotation apply kernel;
+ *   0x400  (with 0x200) shrink the rotation log so every matrix takes the
+ *          split path's overflow fallback (a host-forced test of that path);
+ *   0x1000 / 0x2000 / 0x4000  run only the split path's tridiagonalisation /
+ *          QL / apply phase (L <= 16384): successive calls on one stream with
+ *          one workspace compose the full solve, so each phase can be timed.
+ */
+#define EIGEN_GPU_BATCH_FUSED 0x100
+#define EIGEN_GPU_BATCH_SPLIT 0x200
+#define EIGEN_GPU_BATCH_SPLIT_TINYLOG 0x400
+#define EIGEN_GPU_BATCH_PHASE_TRI 0x1000
+#define EIGEN_GPU_BATCH_PHASE_QL 0x2000
+#define EIGEN_GPU_BATCH_PHASE_APPLY 0x4000
+
 /* Workspace size in bytes for (L, nm, n, m, solver); 0 for invalid arguments. */
 size_t eigen_GPU_batch_bufsize(int L, int nm, int n, int m, int dtype_bytes, int solver);
 
@@ -71,6 +90,33 @@ int eigen_GPU_batch_FP_x(int L, int nm, int n, int m, float *a, float *w, void *
                          hipStream_t stream, int solver, int *info);
 int eigen_GPU_batch_DP_x(int L, int nm, int n, int m, double *a, double *w, void *wk,
                          hipStream_t stream, int solver, int *info);
+
+/*
+ * Uneven batches (no reference symbol: the reference solves one order per call,
+ * eigen_GPU_batch.cu:248-325, so a mixed batch costs one call per order).
+ *
+ *   n    : device int[L], order of matrix k (1 <= n[k] <= nmax).
+ *   nmax : host-side upper bound of the orders, 1 <= nmax <= 512.
+ *   aoff, woff : device int64[L] element offsets of matrix k in a (column-major,
+ *          leading dimension n[k]) and of its n[k] eigenvalues in w; both NULL
+ *          means the packed layout (matrix k follows matrix k-1, offsets are the
+ *          prefix sums of n[j]^2 and n[j]).
+ *   info : device int[L] or NULL; as for the _x entry points, and -3 for a
+ *          matrix whose order is outside 1..nmax (left untouched).
+ *   wk   : device workspace of eigen_GPU_batch_uneven_bufsize(L, nmax, ...)
+ *          bytes.
+ * Solvers: IMTQL2 and TQL2.  Every step is a kernel launched on `stream`
+ * (no allocation, memset or host synchronisation), so a call can be captured
+ * in a HIP graph.  Returns hipSuccess, hipErrorInvalidValue or
+ * hipErrorNotSupported (nothing launched), or the launch error.
+ */
+size_t eigen_GPU_batch_uneven_bufsize(int L, int nmax, int dtype_bytes, int solver);
+int eigen_GPU_batch_uneven_FP(int L, int nmax, const int *n, const long long *aoff,
+                              const long long *woff, float *a, float *w, void *wk,
+                              hipStream_t stream, int solver, int *info);
+int eigen_GPU_batch_uneven_DP(int L, int nmax, const int *n, const long long *aoff,
+                              const long long *woff, double *a, double *w, void *wk,
+                              hipStream_t stream, int solver, int *info);
 
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);
