@@ -24,7 +24,7 @@
 
 namespace eigmi {
 
-constexpr int GL_THREADS = 256;
+constexpr int GL_THREADS = 256;  // default workgroup size (tunable: 256, 512, 1024)
 constexpr int GL_NMAX = 512;
 
 template <typename T>
@@ -36,12 +36,12 @@ struct alignas(16) GlobalLds {
   alignas(16) T beta[GL_NMAX];
   alignas(16) typename vec2<T>::type cs[GL_NMAX];
   alignas(16) int pos[GL_NMAX];
-  alignas(16) T red[2][GL_THREADS / 64];
+  alignas(16) T red[2][16];     // per-wave partial sums (up to 16 waves)
   int ctl[4];                    // wave 0 -> block: sweep bounds and status
 };
 
-template <typename T>
-__device__ __forceinline__ T block_sum(T x, T (&red)[2][GL_THREADS / 64], int slot)
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T x, T (&red)[2][16], int slot)
 {
   x = group_sum<64>(x);
   const int wv = threadIdx.x >> 6;
@@ -49,7 +49,7 @@ __device__ __forceinline__ T block_sum(T x, T (&red)[2][GL_THREADS / 64], int sl
   __syncthreads();
   T s = T(0);
 #pragma unroll
-  for (int q = 0; q < GL_THREADS / 64; ++q) s += red[slot][q];
+  for (int q = 0; q < NT / 64; ++q) s += red[slot][q];
   return s;
 }
 
@@ -85,7 +85,7 @@ __device__ __forceinline__ void chain_row(T *__restrict__ Z, int n, int r, int l
 // Whole solve of one matrix by the workgroup: A (leading dimension nm, order
 // n), eigenvalue output wout[0..n-1], status *infop (may be null), Z an n x n
 // accumulator private to the workgroup.
-template <typename T>
+template <typename T, int NT>
 __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T *__restrict__ wout,
                                              int *__restrict__ infop, int solver, T *__restrict__ Z,
                                              GlobalLds<T> &S)
@@ -96,17 +96,17 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
 
   // ---- 1. symmetrise in place ---------------------------------------------
   for (int c = 0; c < n; ++c)
-    for (int r = c + 1 + t; r < n; r += GL_THREADS) A[r + (size_t)c * nm] = A[c + (size_t)r * nm];
+    for (int r = c + 1 + t; r < n; r += NT) A[r + (size_t)c * nm] = A[c + (size_t)r * nm];
   __syncthreads();
 
   // ---- 2. tridiagonalisation ----------------------------------------------
   for (int c = n - 1; c >= 1; --c) {
-    for (int r = t; r < c; r += GL_THREADS) S.u[r] = A[r + (size_t)c * nm];
+    for (int r = t; r < c; r += NT) S.u[r] = A[r + (size_t)c * nm];
     if (t == 0) S.d[c] = A[c + (size_t)c * nm];
     __syncthreads();
     T part = T(0);
-    for (int r = t; r < c; r += GL_THREADS) part += S.u[r] * S.u[r];
-    const T sig = block_sum(part, S.red, 0);
+    for (int r = t; r < c; r += NT) part += S.u[r] * S.u[r];
+    const T sig = block_sum<NT>(part, S.red, 0);
     const T ulast = S.u[c - 1];
     const T anorm = -sign_(sqrt_(sig), ulast);
     const T ul = ulast - anorm;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
     }
     __syncthreads();
     T p0 = T(0), p1 = T(0);
-    const int r0 = t, r1 = t + GL_THREADS;
+    const int r0 = t, r1 = t + NT;
     if (r0 < c) {
       for (int j = 0; j < c; ++j) {
         const T uj = S.u[j];
@@ -131,7 +131,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
     p0 = (r0 < c) ? p0 * beta : T(0);
     p1 = (r1 < c) ? p1 * beta : T(0);
     const T u0 = (r0 < c) ? S.u[r0] : T(0), u1 = (r1 < c) ? S.u[r1] : T(0);
-    const T alpha = block_sum(p0 * u0 + p1 * u1, S.red, 1) * (beta * T(0.5));
+    const T alpha = block_sum<NT>(p0 * u0 + p1 * u1, S.red, 1) * (beta * T(0.5));
     const T v0 = fma_(alpha, u0, p0), v1 = fma_(alpha, u1, p1);
     if (r0 < c) S.v[r0] = v0;
     if (r1 < c) S.v[r1] = v1;
@@ -153,13 +153,13 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
 
   // ---- 3. Z = H_{n-1} ... H_1 (right-multiplication, row-parallel) --------
   for (int j = 0; j < n; ++j)
-    for (int r = t; r < n; r += GL_THREADS) Z[r + (size_t)j * n] = (r == j) ? T(1) : T(0);
+    for (int r = t; r < n; r += NT) Z[r + (size_t)j * n] = (r == j) ? T(1) : T(0);
   __syncthreads();
   for (int c = n - 1; c >= 1; --c) {
-    for (int r = t; r < c; r += GL_THREADS) S.u[r] = A[r + (size_t)c * nm];
+    for (int r = t; r < c; r += NT) S.u[r] = A[r + (size_t)c * nm];
     __syncthreads();
     const T beta = S.beta[c];
-    for (int r = t; r < n; r += GL_THREADS) {
+    for (int r = t; r < n; r += NT) {
       T tr = T(0);
       for (int j = 0; j < c; ++j) tr = fma_(Z[r + (size_t)j * n], S.u[j], tr);
       tr *= beta;
@@ -169,7 +169,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
   }
 
   // ---- 4. QL: wave 0 drives, everyone applies ---------------------------
-  for (int i = t; i < n; i += GL_THREADS) S.cs[i] = T2{T(1), T(0)};
+  for (int i = t; i < n; i += NT) S.cs[i] = T2{T(1), T(0)};
   __syncthreads();
   int err = 0;
   T shift = T(0), tst1 = T(0);   // explicit-shift state (wave 0)
@@ -286,9 +286,9 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
       const int status = S.ctl[0], m = S.ctl[1];
       if (status == 0) break;
       if (status == 2) { err = l + 1; break; }
-      for (int r = t; r < n; r += GL_THREADS) chain_row(Z, n, r, l, m, S.cs);
+      for (int r = t; r < n; r += NT) chain_row(Z, n, r, l, m, S.cs);
       __syncthreads();
-      for (int i = l + t; i < m; i += GL_THREADS) S.cs[i] = T2{T(1), T(0)};
+      for (int i = l + t; i < m; i += NT) S.cs[i] = T2{T(1), T(0)};
       ++itr;
       tql_started = true;
       __syncthreads();
@@ -300,7 +300,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
   }
 
   if (err) {
-    for (int i = t; i < n; i += GL_THREADS) wout[i] = S.d[i];
+    for (int i = t; i < n; i += NT) wout[i] = S.d[i];
     if (infop && t == 0) *infop = err;
     __syncthreads();
     return;
@@ -336,23 +336,23 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
     }
   }
   __syncthreads();
-  for (int i = t; i < n; i += GL_THREADS) wout[i] = S.d[i];
+  for (int i = t; i < n; i += NT) wout[i] = S.d[i];
   for (int j = 0; j < n; ++j) {
     const int src = S.pos[j];
-    for (int r = t; r < n; r += GL_THREADS) A[r + (size_t)j * nm] = Z[r + (size_t)src * n];
+    for (int r = t; r < n; r += NT) A[r + (size_t)j * nm] = Z[r + (size_t)src * n];
   }
   __syncthreads();
 }
 
-template <typename T>
-__global__ __launch_bounds__(GL_THREADS) void eig_global_kernel(int L, int nm, int n, T *__restrict__ a,
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void eig_global_kernel(int L, int nm, int n, T *__restrict__ a,
                                                                 T *__restrict__ w, int *__restrict__ info,
                                                                 int solver, T *__restrict__ zws)
 {
   __shared__ GlobalLds<T> S;
   T *__restrict__ Z = zws + (size_t)blockIdx.x * (size_t)n * (size_t)n;
   for (long long mat = blockIdx.x; mat < L; mat += gridDim.x)
-    global_solve<T>(a + (size_t)mat * (size_t)nm * (size_t)n, nm, n, w + (size_t)mat * n,
+    global_solve<T, NT>(a + (size_t)mat * (size_t)nm * (size_t)n, nm, n, w + (size_t)mat * n,
                     info ? info + mat : nullptr, solver, Z, S);
 }
 

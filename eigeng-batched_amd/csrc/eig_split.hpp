@@ -203,6 +203,223 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
   }
 }
 
+// Branch-free Givens pair for the lockstep kernel: same values as givens()
+// when (f, g) != 0; otherwise r = 0 and c, s keep their inputs.
+__device__ __forceinline__ bool givens_sel(float f, float g, float &r, float &c, float &s)
+{
+  const float ss = __builtin_fmaf(f, f, g * g);
+  const bool ok = ss != 0.0f;
+  const float ri = rsqrt_nr(ok ? ss : 1.0f);
+  const float cn = g * ri, sn = f * ri;
+  const float h = -0.5f * __builtin_fmaf(cn, cn, __builtin_fmaf(sn, sn, -1.0f));
+  r = ok ? ss * ri : 0.0f;
+  c = ok ? __builtin_fmaf(cn, h, cn) : c;
+  s = ok ? __builtin_fmaf(sn, h, sn) : s;
+  return ok;
+}
+__device__ __forceinline__ bool givens_sel(double f, double g, double &r, double &c, double &s)
+{
+  const double ss = __builtin_fma(f, f, g * g);
+  const bool ok = ss != 0.0;
+  const double ri = rsqrt_nr(ok ? ss : 1.0);
+  r = ok ? ss * ri : 0.0;
+  c = ok ? g * ri : c;
+  s = ok ? f * ri : s;
+  return ok;
+}
+
+// IMTQL2 recurrence, one matrix per lane, with d and e in registers.
+//
+// All lanes of the wave walk the same register index at the same time: each
+// round every lane finds its window [l, m] from a bit mask of negligible
+// off-diagonals, then the chase over i = NB-2 .. 0 is unrolled and a lane acts
+// only on the steps inside its own window.  No array is indexed at run time,
+// so d and e stay in VGPRs.  The per-step arithmetic, the deflation test, the
+// sweep limit and the logs are those of eig_ql_lane_kernel's IMTQL2 branch
+// (imtql2_tiled.hpp:43-164), and the final ordering is the reference's
+// selection sort (tql2_tiled.hpp:149-176), also unrolled over registers.
+template <typename T, int NB>
+__global__ __launch_bounds__(64) void eig_ql_reg_kernel(
+    int L, int n, const T *__restrict__ de, int ld, typename vec2<T>::type *__restrict__ rotlog,
+    int rot_cap, int *__restrict__ swlog, int sw_cap, int *__restrict__ nsw_out, int *__restrict__ status,
+    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info)
+{
+  typedef typename vec2<T>::type T2;
+  static_assert(NB <= 64, "one 64-bit mask per lane");
+  const int lane = threadIdx.x & 63;
+  const long long mat = (long long)blockIdx.x * 64 + lane;
+  const bool live = mat < L;
+  const long long mt = live ? mat : 0;
+  T d[NB], e[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    d[i] = (live && i < n) ? de[(size_t)i * ld + mt] : T(0);
+    e[i] = (live && i < n) ? de[(size_t)(n + i) * ld + mt] : T(0);
+  }
+  T2 *__restrict__ lg = rotlog + (size_t)mt * rot_cap;
+  int *__restrict__ sw = swlog + (size_t)mt * sw_cap;
+  const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
+  // bits n-1 .. 63 always set: e(n-1) = 0 ends every window at n-1 at the latest
+  const uint64_t top = (n - 1 >= 64) ? 0ull : (~0ull << (n - 1));
+  int l = 0, itr = 0, st = 0, nrot = 0, nsw = 0;
+  bool run = live && n > 1;
+
+  int lmin = 0;  // wave minimum of l over running lanes (l never decreases)
+  while (__ballot(run)) {
+    // ---- window of this round ----------------------------------------------
+    // negligible off-diagonals below every running lane's l are never looked
+    // at again, so 8-blocks entirely below lmin are skipped
+    uint64_t small = top;
+#pragma unroll
+    for (int b = 0; b < NB / 8; ++b) {
+      if (8 * b + 8 > lmin) {
+#pragma unroll
+        for (int i = 8 * b; i < 8 * b + 8 && i < NB - 1; ++i) {
+          const bool sm = !(abs_(e[i]) > (abs_(d[i]) + abs_(d[i + 1])) * tol);  // NaN counts as negligible
+          small |= (sm && i < n - 1) ? (1ull << i) : 0ull;
+        }
+      }
+    }
+    if (run && itr > QL_MAX_SWEEP) { st = l + 1; run = false; }
+    const uint64_t cand = ~small & (~0ull << l);
+    if (run && cand == 0ull) run = false;        // every eigenvalue converged
+    int m = 0;
+    if (run) {
+      const int nl = __builtin_ctzll(cand);
+      if (nl != l) itr = 0;
+      l = nl;
+      m = __builtin_ctzll(small & (~0ull << l));
+      if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; run = false; }
+    }
+    const bool sweep = run;
+    // this round's pairs: lo <= i <= hi (empty for idle lanes); wave union
+    int lo = sweep ? l : NB + 8, hi = sweep ? m - 1 : -1;
+    int lo_w = lo, hi_w = hi;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const int lo2 = __shfl_xor(lo_w, off), hi2 = __shfl_xor(hi_w, off);
+      lo_w = lo2 < lo_w ? lo2 : lo_w;
+      hi_w = hi2 > hi_w ? hi2 : hi_w;
+    }
+    lo_w = __builtin_amdgcn_readfirstlane(lo_w);
+    hi_w = __builtin_amdgcn_readfirstlane(hi_w);
+    lmin = lo_w;
+    // d(l), e(l), d(l+1), d(m) of each lane (per-lane indices: select chain)
+    T dl = T(0), el = T(1), dl1 = T(0), dm = T(0);
+#pragma unroll
+    for (int b = 0; b < NB / 8; ++b) {
+      if (8 * b + 8 > lo_w && 8 * b <= hi_w + 1) {
+#pragma unroll
+        for (int i = 8 * b; i < 8 * b + 8; ++i) {
+          if (i < NB - 1) {
+            const bool at = (i == lo);
+            dl = at ? d[i] : dl;
+            el = at ? e[i] : el;
+            dl1 = at ? d[i + 1] : dl1;
+          }
+          dm = (i == hi + 1) ? d[i] : dm;
+        }
+      }
+    }
+    T r = T(0);
+    if (sweep) {
+      const T f0 = (dl1 - dl) / (el + el);
+      const T g0 = hypot1_c(f0);
+      r = (dm - dl) + el / (f0 + sign_(g0, f0));
+    }
+    // ---- chase, i = m-1 down to l (lockstep over the wave's union) ----------
+    // the window bounds go through an empty asm so that the step predicates
+    // are formed where they are used instead of being kept from above
+    asm volatile("" : "+v"(lo), "+v"(hi));
+    const unsigned span = sweep ? (unsigned)(hi - lo) : 0u;   // idle lanes: i - lo < 0 fails
+    T s = T(1), c = T(1), delta = T(0);
+    int ilo = l;
+#pragma unroll
+    for (int b = NB / 8 - 1; b >= 0; --b) {
+      if (8 * b <= hi_w && 8 * b + 7 >= lo_w) {
+#pragma unroll
+        for (int i = 8 * b + 7; i >= 8 * b; --i) {
+          if (i <= NB - 2) {
+            const bool act = (unsigned)(i - lo) <= span;
+            const T ei = e[i];
+            const T f = s * ei, bb = c * ei;
+            T rn, cn = c, sn = s;
+            const bool ok = givens_sel(f, r, rn, cn, sn);
+            const bool go = act && ok, brk = act && !ok;
+            const T dix = d[i + 1] - delta;
+            const T q = (d[i] - dix) * sn + T(2) * cn * bb;
+            const T dx1 = fma_(sn, q, dix);
+            if (go) lg[nrot] = T2{cn, sn};
+            nrot += go ? 1 : 0;
+            // e(i+1) = r (e(m) ends at 0; 0 after an underflow break), d(i+1) = dx1
+            e[i + 1] = act ? ((i == hi || brk) ? T(0) : rn) : e[i + 1];
+            d[i + 1] = act ? (go ? dx1 : dix) : d[i + 1];
+            delta = go ? dx1 - dix : delta;
+            r = go ? cn * q - bb : r;
+            c = go ? cn : c;
+            s = go ? sn : s;
+            // end of the window: d(l) -= delta, e(l) = r
+            const bool fin = go && (i == lo);
+            d[i] = fin ? d[i] - delta : d[i];
+            e[i] = fin ? r : e[i];
+            ilo = brk ? i + 1 : ilo;
+            lo = brk ? NB + 8 : lo;    // a break ends this lane's sweep
+          }
+        }
+      }
+    }
+    if (sweep) {
+      sw[nsw++] = sw_code(l, m, ilo);
+      ++itr;
+    }
+  }
+
+  if (!live) return;
+  status[mat] = st;
+  nsw_out[mat] = nsw;
+  if (st == ST_OVERFLOW) return;                 // solved again by the fused kernel
+  if (info) info[mat] = st;
+  if (st != 0) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i < n) w[(size_t)mat * n + i] = d[i];
+    return;
+  }
+  // selection sort with the reference's rule; e now holds the positions
+#pragma unroll
+  for (int i = 0; i < NB; ++i) e[i] = T(i);
+#pragma unroll
+  for (int l2 = 0; l2 < NB - 1; ++l2) {
+    if (l2 < n - 1) {
+      T dm = d[l2], pm = e[l2];
+      int im = l2;
+#pragma unroll
+      for (int j = l2 + 1; j < NB; ++j) {
+        const bool lt = (j < n) && (dm > d[j]);
+        dm = lt ? d[j] : dm;
+        pm = lt ? e[j] : pm;
+        im = lt ? j : im;
+      }
+      const T dl2 = d[l2], pl2 = e[l2];
+#pragma unroll
+      for (int j = l2 + 1; j < NB; ++j) {
+        const bool hit = (j == im);
+        d[j] = hit ? dl2 : d[j];
+        e[j] = hit ? pl2 : e[j];
+      }
+      d[l2] = dm;
+      e[l2] = pm;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (i < n) {
+      w[(size_t)mat * n + i] = d[i];
+      slot[(size_t)mat * n + (int)e[i]] = i;
+    }
+  }
+}
+
 template <typename T, int NB>
 struct alignas(16) ApplyLds {
   alignas(16) typename vec2<T>::type cs[NB];

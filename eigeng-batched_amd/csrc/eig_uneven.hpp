@@ -185,7 +185,7 @@ __global__ __launch_bounds__(GL_THREADS) void eig_global_list_kernel(
     if (t0 >= end) break;
     const int mat = list[t0];
     const int n = __builtin_amdgcn_readfirstlane(nv[mat]);
-    global_solve<T>(a + aoff[mat], n, n, w + woff[mat], info ? info + mat : nullptr, solver, Z, S);
+    global_solve<T, GL_THREADS>(a + aoff[mat], n, n, w + woff[mat], info ? info + mat : nullptr, solver, Z, S);
   }
 }
 

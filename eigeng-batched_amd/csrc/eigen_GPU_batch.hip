@@ -34,6 +34,9 @@ constexpr int SPLIT_MIN_L = 2048; // split path used from this batch size on (n 
 // path; FLAG_TINYLOG shrinks the split path's rotation log so that every
 // matrix takes the overflow fallback (a host-forced test of that path)
 constexpr int FLAG_FUSED = 0x100, FLAG_SPLIT = 0x200, FLAG_TINYLOG = 0x400;
+// split path: run the QL recurrence with d, e in LDS (eig_ql_lane_kernel) even
+// where the register lockstep kernel applies (a host-side switch for tests)
+constexpr int FLAG_QL_LDS = 0x800;
 // split path phase selection: a call with some of these bits runs only those
 // phases (tridiagonalisation + Q, QL recurrence, rotation apply + fallback);
 // the phases hand over through the workspace, so consecutive calls with the
@@ -122,9 +125,16 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
     const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
     if (ph & FLAG_PH_TRI)
       hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, de, lc);
-    if (ph & FLAG_PH_QL)
-      hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap, sw,
-                         scap, nsw, st, slot, wc, ic, solver & 0xff);
+    if (ph & FLAG_PH_QL) {
+      // registers hold d and e (2 NB values per lane): FP32 up to NB = 64, FP64 up to 32
+      const bool reg = (solver & 0xff) == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS) && NB * sizeof(T) <= 256;
+      if (reg)
+        hipLaunchKernelGGL((eig_ql_reg_kernel<T, (NB * sizeof(T) <= 256 ? NB : 8)>), dim3(lb), dim3(64), 0, s,
+                           cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic);
+      else
+        hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap,
+                           sw, scap, nsw, st, slot, wc, ic, solver & 0xff);
+    }
     if (ph & FLAG_PH_APPLY) {
       hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
                          sw, scap, nsw, st, slot);
@@ -145,12 +155,33 @@ hipError_t launch_jacobi(int L, int nm, int n, T *a, T *w, int *info, hipStream_
   return hipGetLastError();
 }
 
+// Workgroup kernel tuning (bits of `solver`): workgroup size code at
+// GL_NT_SHIFT (1: 256, 2: 512, 3: 1024 threads) and resident-workgroup count
+// code at GL_GRID_SHIFT (g: 128 g workgroups, at most 1024); 0 picks the
+// defaults.  Fewer, wider workgroups keep the matrices being worked on (and
+// their accumulators) inside the on-chip caches.
+constexpr int GL_NT_SHIFT = 16, GL_GRID_SHIFT = 20;
+
 template <typename T>
 hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
 {
   T *zws = (T *)((char *)wk + WS_HEAD);
-  hipLaunchKernelGGL((eig_global_kernel<T>), dim3(global_grid(L)), dim3(GL_THREADS), 0, s,
-                     L, nm, n, a, w, info, solver, zws);
+  const int ntc = (solver >> GL_NT_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
+  int grid = global_grid(L);
+  if (gc) grid = (128 * gc < grid) ? 128 * gc : grid;
+  const int base = solver & 0xff;
+  switch (ntc) {
+  case 2:
+    hipLaunchKernelGGL((eig_global_kernel<T, 512>), dim3(grid), dim3(512), 0, s, L, nm, n, a, w, info, base, zws);
+    break;
+  case 3:
+    hipLaunchKernelGGL((eig_global_kernel<T, 1024>), dim3(grid), dim3(1024), 0, s, L, nm, n, a, w, info, base,
+                       zws);
+    break;
+  default:
+    hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(grid), dim3(GL_THREADS), 0, s, L, nm, n, a, w,
+                       info, base, zws);
+  }
   return hipGetLastError();
 }
 
@@ -180,7 +211,7 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
   if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
   if (use_wave_kernel(n, (int)sizeof(T))) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
-  if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, base, wk, s);
+  if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
   return hipErrorNotSupported;
 }
 

@@ -22,7 +22,9 @@ LIB_PATH = os.path.join(HERE, "libeigenGbatch.so")
 IMTQL2, TQL2, JACOBI = 0, 1, 2
 # tuning flags OR-ed into the solver id of the _x entry points
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
-PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG}
+FLAG_QL_LDS = 0x800
+PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG,
+         "split-lds": FLAG_SPLIT | FLAG_QL_LDS}
 # split-path phases (a call with a subset runs only those kernels; the phases
 # hand over through the workspace, so tri -> ql -> apply calls on one stream
 # and one workspace compose the full solve, for batches of at most 16384)
@@ -92,7 +94,9 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     abi=True calls the reference-compatible eigen_GPU_batch_FP/_DP symbol.
     path selects the n <= 64 implementation: "auto", "fused" (one kernel),
     "split" (lane-per-matrix QL recurrence) or "split-fallback" (split with the
-    rotation log shrunk so that every matrix takes the overflow fallback).
+    rotation log shrunk so that every matrix takes the overflow fallback) or
+    "split-lds" (split with the QL recurrence's d, e kept in LDS rather than in
+    registers).
     phases (split path only): iterable of "tri", "ql", "apply" to run.
     """
     import torch

@@ -89,7 +89,7 @@ def test_sym_rand_vs_oracle(n, dtype):
 SPLIT_SIZES = [1, 2, 3, 5, 8, 9, 16, 17, 31, 32, 33, 48, 63, 64]
 
 
-@pytest.mark.parametrize("path", ["split", "split-fallback"])
+@pytest.mark.parametrize("path", ["split", "split-fallback", "split-lds"])
 @pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", SPLIT_SIZES)
@@ -105,6 +105,24 @@ def test_split_path_vs_oracle(n, dtype, solver, path):
     if nm > n:
         assert np.isnan(z[:, :, n:]).all()
     check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [2, 7, 16, 31, 48, 64])
+def test_ql_register_and_lds_kernels_agree(n, dtype):
+    """The register lockstep QL kernel and the LDS lane kernel run the same
+    per-step arithmetic: eigenvalues and eigenvectors agree to a few ulps
+    (bit-identical unless the compiler contracts an expression differently)."""
+    print(f"[parity] ql reg vs lds n={n} dtype={np.dtype(dtype).name}", flush=True)
+    a0, _ = wl.letkf_y(150, k=n, p=3 * n + 10, rho=1.05, sigma=0.1, dtype=dtype, seed=100 + n)
+    z1, w1, i1 = run_gpu(a0, path="split")
+    z2, w2, i2 = run_gpu(a0, path="split-lds")
+    assert (i1 == 0).all() and (i2 == 0).all()
+    eps = np.finfo(dtype).eps
+    scale = np.abs(w2).max(axis=1, keepdims=True)
+    assert np.max(np.abs(w1 - w2) / scale) <= 16 * eps
+    print(f"  bitwise equal: w {np.array_equal(w1, w2)} z {np.array_equal(z1, z2)}", flush=True)
+    check_batch(a0, z1, w1, dtype, w2, z2)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
