@@ -82,18 +82,14 @@ __device__ __forceinline__ void chain_row(T *__restrict__ Z, int n, int r, int l
   Z[r + (size_t)l * n] = h1;
 }
 
-// Whole solve of one matrix by the workgroup: A (leading dimension nm, order
-// n), eigenvalue output wout[0..n-1], status *infop (may be null), Z an n x n
-// accumulator private to the workgroup.
+// Symmetrise A in place (lower <- upper) and reduce it to tridiagonal form by
+// Householder reflections (tred1_tiled.hpp:42-212 convention): on return
+// S.d / S.e hold the diagonal and off-diagonal (e(n-1) = 0), S.beta[c] the
+// scale of reflector c, and u_c sits in A(0:c-1, c).
 template <typename T, int NT>
-__device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T *__restrict__ wout,
-                                             int *__restrict__ infop, int solver, T *__restrict__ Z,
-                                             GlobalLds<T> &S)
+__device__ __forceinline__ void global_tridiag(T *__restrict__ A, int nm, int n, GlobalLds<T> &S)
 {
-  typedef typename vec2<T>::type T2;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
-
+  const int t = threadIdx.x;
   // ---- 1. symmetrise in place ---------------------------------------------
   for (int c = 0; c < n; ++c)
     for (int r = c + 1 + t; r < n; r += NT) A[r + (size_t)c * nm] = A[c + (size_t)r * nm];
@@ -150,6 +146,21 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
     __syncthreads();
   }
   if (t == 0) { S.d[0] = A[0]; S.e[n - 1] = T(0); }
+}
+
+// Whole solve of one matrix by the workgroup: A (leading dimension nm, order
+// n), eigenvalue output wout[0..n-1], status *infop (may be null), Z an n x n
+// accumulator private to the workgroup.
+template <typename T, int NT>
+__device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T *__restrict__ wout,
+                                             int *__restrict__ infop, int solver, T *__restrict__ Z,
+                                             GlobalLds<T> &S)
+{
+  typedef typename vec2<T>::type T2;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
+
+  global_tridiag<T, NT>(A, nm, n, S);
 
   // ---- 3. Z = H_{n-1} ... H_1 (right-multiplication, row-parallel) --------
   for (int j = 0; j < n; ++j)
@@ -344,15 +355,19 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
   __syncthreads();
 }
 
+// `only` (may be null): solve just the matrices flagged ST_OVERFLOW there (the
+// fallback pass of the split path).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void eig_global_kernel(int L, int nm, int n, T *__restrict__ a,
                                                                 T *__restrict__ w, int *__restrict__ info,
-                                                                int solver, T *__restrict__ zws)
+                                                                int solver, T *__restrict__ zws,
+                                                                const int *__restrict__ only)
 {
   __shared__ GlobalLds<T> S;
   T *__restrict__ Z = zws + (size_t)blockIdx.x * (size_t)n * (size_t)n;
   for (long long mat = blockIdx.x; mat < L; mat += gridDim.x)
-    global_solve<T, NT>(a + (size_t)mat * (size_t)nm * (size_t)n, nm, n, w + (size_t)mat * n,
+    if (!only || only[mat] == ST_OVERFLOW)
+      global_solve<T, NT>(a + (size_t)mat * (size_t)nm * (size_t)n, nm, n, w + (size_t)mat * n,
                     info ? info + mat : nullptr, solver, Z, S);
 }
 

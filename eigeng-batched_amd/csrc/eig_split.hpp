@@ -31,8 +31,10 @@ namespace eigmi {
 __host__ __device__ constexpr int rot_cap_of(int n) { return (3 * n * n) / 2 + 16; }
 __host__ __device__ constexpr int sw_cap_of(int n) { return 3 * n + 8; }
 
-// sweep code: l | m << 8 | lowest pair << 16  (n <= 128)
-__device__ __forceinline__ int sw_code(int l, int m, int ilo) { return l | (m << 8) | (ilo << 16); }
+// sweep code: l | m << 10 | lowest pair << 20  (n <= 1024)
+__device__ __forceinline__ int sw_code(int l, int m, int ilo) { return l | (m << 10) | (ilo << 20); }
+__device__ __forceinline__ int sw_m(int code) { return (code >> 10) & 1023; }
+__device__ __forceinline__ int sw_ilo(int code) { return (code >> 20) & 1023; }
 
 template <typename T, int NB>
 __global__ __launch_bounds__(64) void eig_tri_kernel(int L, int nm, int n, const T *__restrict__ a,
@@ -59,17 +61,20 @@ __global__ __launch_bounds__(64) void eig_tri_kernel(int L, int nm, int n, const
   }
 }
 
-template <typename T, int NMAX>
+// NLANE matrices per 64-lane workgroup (the rest of the lanes idle): the LDS
+// holds 2 NMAX NLANE values, so large orders use fewer lanes per wave.
+template <typename T, int NMAX, int NLANE = 64>
 __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
     int L, int n, const T *__restrict__ de, int ld, typename vec2<T>::type *__restrict__ rotlog,
     int rot_cap, int *__restrict__ swlog, int sw_cap, int *__restrict__ nsw_out, int *__restrict__ status,
     int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int solver)
 {
   typedef typename vec2<T>::type T2;
-  __shared__ T dS[NMAX][64];
-  __shared__ T eS[NMAX][64];
+  __shared__ T dS[NMAX][NLANE];
+  __shared__ T eS[NMAX][NLANE];
   const int lane = threadIdx.x & 63;
-  const long long mat = (long long)blockIdx.x * 64 + lane;
+  if (lane >= NLANE) return;
+  const long long mat = (long long)blockIdx.x * NLANE + lane;
   if (mat >= L) return;
   for (int i = 0; i < n; ++i) {
     dS[i][lane] = de[(size_t)i * ld + mat];
@@ -459,19 +464,18 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
   int base = 0;
   T2 q = T2{T(1), T(0)};
   {
-    const int m = (code >> 8) & 255, ilo = code >> 16;
+    const int m = sw_m(code), ilo = sw_ilo(code);
     if (nsw > 0 && k < m - ilo) q = lg[k];
   }
   for (int s = 0; s < nsw; ++s) {
-    const int l = code & 255, m = (code >> 8) & 255, ilo = code >> 16;
-    (void)l;
+    const int m = sw_m(code), ilo = sw_ilo(code);
     const int cnt = m - ilo;
     if (k < cnt) S.cs[m - 1 - k] = q;
     base += cnt;
     // prefetch sweep s+1
     const int code2 = (s + 1 < nsw) ? sw[s + 1] : 0;
     {
-      const int m2 = (code2 >> 8) & 255, ilo2 = code2 >> 16;
+      const int m2 = sw_m(code2), ilo2 = sw_ilo(code2);
       if (s + 1 < nsw && k < m2 - ilo2) q = lg[base + k];
     }
     wsync();

@@ -1,0 +1,388 @@
+// eig_split_large.hpp - split solver for the orders beyond the register kernels
+// (FP32 128 < n <= 512, FP64 64 < n <= 512).
+//
+//   eig_tri_large_kernel   (workgroup per matrix, persistent grid): copy a into
+//                          a workspace slot, Householder tridiagonalisation there
+//                          (u_c in W(0:c-1, c), beta_c to bws), d / e to the
+//                          matrix-minor layout read by the QL kernel; a itself
+//                          is not written, so the workgroup fallback can still
+//                          solve any matrix whose rotation log overflows.
+//   eig_ql_lane_kernel     (eig_split.hpp, NLANE matrices per wave): the QL
+//                          recurrence with rotation and sweep logs, sort, w.
+//   eig_apply_large_kernel (workgroup per 64-row block of a matrix): rows of
+//                          Q = H_{n-1} ... H_1 formed in registers, then every
+//                          logged sweep replayed on them, then the permuted store.
+//
+// Apply kernel layout: lane = row of the block, wave q = 32-column chunk
+// [32q, 32q+31] held in registers.  Rotations act on columns, so rows are
+// independent, but one sweep chains through the columns from the top of its
+// window down: wave q handles the pairs i in [32q-1, 32q+30] (pair i finalises
+// column i+1, i.e. the wave's own columns).  Pair 32q-1 needs the top column of
+// wave q-1 (published in LDS after that wave's previous sweep) and hands the
+// pending value down to wave q-1 (LDS carry).  With step T(q, s) = 2 s +
+// (NCH-1-q) for sweep s, wave q-1 has finished sweep s-1 one step before wave
+// q starts sweep s, and starts sweep s one step after it: one workgroup barrier
+// per step keeps the chain exact.  The rotation of a pair and its arithmetic are
+// those of apply_sweep_rotations (the n <= 64 apply kernel).
+#pragma once
+#include "common.hpp"
+#include "eig_global.hpp"
+#include "eig_split.hpp"
+
+namespace eigmi {
+
+constexpr int AL_CW = 32;  // columns per wave in the apply kernel
+
+// ---- blocked Householder tridiagonalisation ---------------------------------
+// The reduction of column c needs A11 = A(0:c-1, 0:c-1) with the rank-2
+// updates of all earlier columns applied.  In panels of NBP columns those
+// updates are kept as the vectors U_k, V_k (zero beyond their length) and
+// folded in analytically (the dlatrd arrangement):
+//   column c      = A(:, c) + sum_k (U_k V_k(c) + V_k U_k(c))
+//   p = beta A11 u = beta (A u + sum_k (U_k (V_k . u) + V_k (U_k . u)))
+// and A11 itself is updated once per panel.  Per column this is one read pass
+// over A11 instead of a read pass plus a read-modify-write pass; the
+// Householder convention (anorm, beta, u_c stored in A(0:c-1, c), d, e) is the
+// unblocked one's.
+template <typename T, int NBP, int NMAXT>
+struct alignas(16) TriBlkLds {
+  alignas(16) T U[NBP][NMAXT];
+  alignas(16) T V[NBP][NMAXT];
+  alignas(16) T u[NMAXT];
+  alignas(16) T red[16][2 * NBP + 2];
+  alignas(16) T d[NMAXT];
+  alignas(16) T e[NMAXT];
+  alignas(16) T beta[NMAXT];
+};
+
+// Sum over the workgroup of K per-thread values (K <= 2 NBP + 2); every thread
+// gets the K sums.
+template <int NT, int K, int NBP, typename T, int NMAXT>
+__device__ __forceinline__ void block_sums(T (&x)[K], int cnt, TriBlkLds<T, NBP, NMAXT> &S)
+{
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (i < cnt) {
+      const T v = group_sum<64>(x[i]);
+      if (lane == 0) S.red[wv][i] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (i < cnt) {
+      T sum = T(0);
+#pragma unroll
+      for (int q = 0; q < NT / 64; ++q) sum += S.red[q][i];
+      x[i] = sum;
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int NT, int NBP, int NMAXT>
+__device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, NBP, NMAXT> &S)
+{
+  static_assert(2 * NT >= NMAXT, "two rows per thread cover the matrix");
+  const int t = threadIdx.x;
+  const int r0 = t, r1 = t + NT;
+  // symmetrise (lower <- upper)
+  for (int c = 0; c < n; ++c)
+    for (int r = c + 1 + t; r < n; r += NT) A[r + (size_t)c * nm] = A[c + (size_t)r * nm];
+  for (int i = t; i < NMAXT; i += NT) {
+#pragma unroll
+    for (int k = 0; k < NBP; ++k) { S.U[k][i] = T(0); S.V[k][i] = T(0); }
+  }
+  __syncthreads();
+
+  for (int ctop = n - 1; ctop >= 1; ctop -= NBP) {
+    const int nbk = (NBP < ctop) ? NBP : ctop;
+    for (int k = 0; k < nbk; ++k) {
+      const int c = ctop - k;
+      // 1. column c of the updated matrix (rows < c) and its diagonal
+      T uc0 = T(0), uc1 = T(0);
+      {
+        T ukc[NBP], vkc[NBP];
+#pragma unroll
+        for (int kk = 0; kk < NBP; ++kk) { ukc[kk] = S.U[kk][c]; vkc[kk] = S.V[kk][c]; }
+        if (r0 <= c) {
+          T acc = A[r0 + (size_t)c * nm];
+#pragma unroll
+          for (int kk = 0; kk < NBP; ++kk)
+            if (kk < k) acc = fma_(S.U[kk][r0], vkc[kk], fma_(S.V[kk][r0], ukc[kk], acc));
+          if (r0 < c) uc0 = acc; else S.d[c] = acc;
+        }
+        if (r1 <= c) {
+          T acc = A[r1 + (size_t)c * nm];
+#pragma unroll
+          for (int kk = 0; kk < NBP; ++kk)
+            if (kk < k) acc = fma_(S.U[kk][r1], vkc[kk], fma_(S.V[kk][r1], ukc[kk], acc));
+          if (r1 < c) uc1 = acc; else S.d[c] = acc;
+        }
+      }
+      if (r0 < c) S.u[r0] = uc0;
+      if (r1 < c) S.u[r1] = uc1;
+      T sg[1] = {uc0 * uc0 + uc1 * uc1};
+      block_sums<NT, 1, NBP>(sg, 1, S);
+      const T ulast = S.u[c - 1];
+      const T anorm = -sign_(sqrt_(sg[0]), ulast);
+      const T ul = ulast - anorm;
+      const T beta = T(1) / nz(ul * anorm);
+      if (r0 == c - 1) uc0 = ul;
+      if (r1 == c - 1) uc1 = ul;
+      // u_c is stored in A(0:c-1, c) (with u(c-1) = ul), as in the unblocked form
+      if (r0 < c) A[r0 + (size_t)c * nm] = uc0;
+      if (r1 < c) A[r1 + (size_t)c * nm] = uc1;
+      __syncthreads();
+      if (t == 0) {
+        S.u[c - 1] = ul;
+        S.e[c - 1] = anorm;
+        S.beta[c] = beta;
+      }
+      __syncthreads();
+      // 2. dots of the panel vectors with u, then p
+      T dd[2 * NBP];
+#pragma unroll
+      for (int kk = 0; kk < 2 * NBP; ++kk) dd[kk] = T(0);
+#pragma unroll
+      for (int kk = 0; kk < NBP; ++kk) {
+        if (kk < k) {
+          if (r0 < c) { dd[2 * kk] = S.V[kk][r0] * uc0; dd[2 * kk + 1] = S.U[kk][r0] * uc0; }
+          if (r1 < c) {
+            dd[2 * kk] = fma_(S.V[kk][r1], uc1, dd[2 * kk]);
+            dd[2 * kk + 1] = fma_(S.U[kk][r1], uc1, dd[2 * kk + 1]);
+          }
+        }
+      }
+      block_sums<NT, 2 * NBP, NBP>(dd, 2 * k, S);
+      T p0 = T(0), p1 = T(0);
+      if (r0 < c) {
+        for (int j = 0; j < c; ++j) {
+          const T uj = S.u[j];
+          p0 = fma_(A[r0 + (size_t)j * nm], uj, p0);
+          if (r1 < c) p1 = fma_(A[r1 + (size_t)j * nm], uj, p1);
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < NBP; ++kk) {
+        if (kk < k) {
+          if (r0 < c) p0 = fma_(S.U[kk][r0], dd[2 * kk], fma_(S.V[kk][r0], dd[2 * kk + 1], p0));
+          if (r1 < c) p1 = fma_(S.U[kk][r1], dd[2 * kk], fma_(S.V[kk][r1], dd[2 * kk + 1], p1));
+        }
+      }
+      p0 = (r0 < c) ? p0 * beta : T(0);
+      p1 = (r1 < c) ? p1 * beta : T(0);
+      T al[1] = {p0 * uc0 + p1 * uc1};
+      block_sums<NT, 1, NBP>(al, 1, S);
+      const T alpha = al[0] * (beta * T(0.5));
+      if (r0 < NMAXT) { S.U[k][r0] = (r0 < c) ? uc0 : T(0); S.V[k][r0] = (r0 < c) ? fma_(alpha, uc0, p0) : T(0); }
+      if (r1 < NMAXT) { S.U[k][r1] = (r1 < c) ? uc1 : T(0); S.V[k][r1] = (r1 < c) ? fma_(alpha, uc1, p1) : T(0); }
+      __syncthreads();
+    }
+    // 3. panel update of the trailing block A(0:clow-1, 0:clow-1)
+    const int clow = ctop - nbk + 1;
+    if (clow > 1 || (clow == 1)) {
+      T ur0[NBP], vr0[NBP], ur1[NBP], vr1[NBP];
+#pragma unroll
+      for (int kk = 0; kk < NBP; ++kk) {
+        ur0[kk] = (r0 < clow && kk < nbk) ? S.U[kk][r0] : T(0);
+        vr0[kk] = (r0 < clow && kk < nbk) ? S.V[kk][r0] : T(0);
+        ur1[kk] = (r1 < clow && kk < nbk) ? S.U[kk][r1] : T(0);
+        vr1[kk] = (r1 < clow && kk < nbk) ? S.V[kk][r1] : T(0);
+      }
+      if (r0 < clow) {
+        for (int j = 0; j < clow; ++j) {
+          T a0 = A[r0 + (size_t)j * nm];
+          T a1 = (r1 < clow) ? A[r1 + (size_t)j * nm] : T(0);
+#pragma unroll
+          for (int kk = 0; kk < NBP; ++kk) {
+            if (kk < nbk) {
+              const T uj = S.U[kk][j], vj = S.V[kk][j];
+              a0 = fma_(ur0[kk], vj, fma_(vr0[kk], uj, a0));
+              a1 = fma_(ur1[kk], vj, fma_(vr1[kk], uj, a1));
+            }
+          }
+          A[r0 + (size_t)j * nm] = a0;
+          if (r1 < clow) A[r1 + (size_t)j * nm] = a1;
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < NMAXT; i += NT) {
+#pragma unroll
+      for (int k = 0; k < NBP; ++k) { S.U[k][i] = T(0); S.V[k][i] = T(0); }
+    }
+    __syncthreads();
+  }
+  if (t == 0) { S.d[0] = A[0]; S.e[n - 1] = T(0); }
+  __syncthreads();
+}
+
+template <typename T, int NT, int NBP, int NMAXT>
+__global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n, const T *__restrict__ a,
+                                                           T *__restrict__ W, T *__restrict__ de, int ld,
+                                                           T *__restrict__ bws)
+{
+  __shared__ TriBlkLds<T, NBP, NMAXT> S;
+  const int t = threadIdx.x;
+  for (long long mat = blockIdx.x; mat < L; mat += gridDim.x) {
+    const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
+    T *__restrict__ Wm = W + (size_t)mat * (size_t)n * (size_t)n;
+    for (int c = 0; c < n; ++c)
+      for (int r = t; r <= c; r += NT) Wm[r + (size_t)c * n] = A[r + (size_t)c * nm];
+    __syncthreads();
+    blocked_tridiag<T, NT, NBP, NMAXT>(Wm, n, n, S);
+    for (int i = t; i < n; i += NT) {
+      de[(size_t)i * ld + mat] = S.d[i];
+      de[(size_t)(n + i) * ld + mat] = S.e[i];
+      bws[(size_t)mat * n + i] = S.beta[i];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int NCH>
+struct alignas(16) ApplyLargeLds {
+  alignas(16) T red[2][NCH][64];                 // per-wave partial dot products (phase A)
+  alignas(16) T topv[NCH][64];                   // wave q's top column after its last sweep
+  alignas(16) T carry[NCH][64];                  // pending chain value handed to wave q
+  alignas(16) T us[NCH][AL_CW];                  // reflector entries of wave q's columns
+  alignas(16) typename vec2<T>::type rs[NCH][AL_CW];  // rotations of wave q's current sweep
+  alignas(16) int pos[NCH][AL_CW];
+};
+
+template <typename T, int NCH>
+__global__ __launch_bounds__(64 * NCH) void eig_apply_large_kernel(
+    int L, int nm, int n, T *__restrict__ a, const T *__restrict__ W, const T *__restrict__ bws,
+    const typename vec2<T>::type *__restrict__ rotlog, int rot_cap, const int *__restrict__ swlog, int sw_cap,
+    const int *__restrict__ nsw_in, const int *__restrict__ status, const int *__restrict__ slot)
+{
+  typedef typename vec2<T>::type T2;
+  constexpr int CW = AL_CW;
+  __shared__ ApplyLargeLds<T, NCH> S;
+  const int nrb = (n + 63) >> 6;
+  const long long mat = blockIdx.x / nrb;
+  const int rb = blockIdx.x - (int)(mat * nrb);
+  if (mat >= L) return;
+  if (status[mat] != 0) return;                      // overflow: solved by the fallback
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int r = rb * 64 + lane;                      // row of Q handled by this lane
+  const int c0 = q * CW;                             // first column of this wave
+  const T *__restrict__ Wm = W + (size_t)mat * (size_t)n * (size_t)n;
+  const T *__restrict__ beta = bws + (size_t)mat * n;
+
+  T x[CW];
+#pragma unroll
+  for (int j = 0; j < CW; ++j) x[j] = (r == c0 + j) ? T(1) : T(0);
+
+  // ---- A. rows of Q = H_{n-1} ... H_1: x <- x H_c, c = n-1 down to 1 --------
+  // (the right-multiplication of wave_front, split over the column chunks)
+  T unext = (lane < CW && c0 + lane < n - 1) ? Wm[c0 + lane + (size_t)(n - 1) * n] : T(0);
+  for (int c = n - 1; c >= 1; --c) {
+    const T ucur = (c0 + lane < c) ? unext : T(0);
+    if (c > 1 && lane < CW) unext = (c0 + lane < c - 1) ? Wm[c0 + lane + (size_t)(c - 1) * n] : T(0);
+    T part = T(0);
+    if (c0 < c) {
+      if (lane < CW) S.us[q][lane] = ucur;
+      wsync();
+#pragma unroll
+      for (int j = 0; j < CW; ++j) part = fma_(x[j], S.us[q][j], part);
+    }
+    S.red[c & 1][q][lane] = part;
+    __syncthreads();
+    T tsum = T(0);
+#pragma unroll
+    for (int w2 = 0; w2 < NCH; ++w2) tsum += S.red[c & 1][w2][lane];
+    tsum *= beta[c];
+    if (c0 < c) {
+#pragma unroll
+      for (int j = 0; j < CW; ++j) x[j] = fma_(tsum, S.us[q][j], x[j]);
+    }
+    wsync();
+  }
+
+  // ---- B. the logged sweeps, wave-pipelined over the column chunks ----------
+  const T2 *__restrict__ lg = rotlog + (size_t)mat * rot_cap;
+  const int *__restrict__ sw = swlog + (size_t)mat * sw_cap;
+  const int nsw = nsw_in[mat];
+  // this wave's pairs: plo .. phi (pair i finalises column i+1)
+  const int phi = c0 + CW - 2;
+  const int plo = (c0 - 1 > 0) ? c0 - 1 : 0;
+  int base = 0;                                      // log offset of the wave's next sweep
+  int code = (nsw > 0) ? sw[0] : 0;
+  S.topv[q][lane] = x[CW - 1];
+  __syncthreads();
+  const int nsteps = (nsw > 0) ? 2 * (nsw - 1) + NCH : 0;
+  int s = 0;                                         // next sweep of this wave
+  for (int step = 0; step < nsteps; ++step) {
+    const int rel = step - (NCH - 1 - q);
+    if (rel >= 0 && (rel & 1) == 0 && s < nsw) {
+      const int m = sw_m(code), ilo = sw_ilo(code);
+      const int cnt = m - ilo;
+      const int top = (m - 1 < phi) ? m - 1 : phi;
+      const int bot = (ilo > plo) ? ilo : plo;
+      const bool enters = (m - 1 > phi) && (ilo <= phi + 1);
+      // rotations of this wave's pairs (pair i at log position base + m-1-i)
+      if (top >= bot && lane < CW) {
+        const int i = top - lane;
+        if (i >= bot) S.rs[q][i - plo] = lg[base + (m - 1 - i)];
+      }
+      wsync();
+      T h1 = T(0);
+      bool live = false;
+      if (enters) {
+        h1 = S.carry[q][lane];
+        live = true;
+        if (ilo == phi + 1) {                        // the chain ended on the pair above
+          x[CW - 1] = h1;
+          live = false;
+        }
+      } else if (m - 1 >= plo && m - 1 <= phi) {
+        // the window's top column m is this wave's column m - c0
+#pragma unroll
+        for (int j = 0; j < CW; ++j)
+          if (c0 + j == m) h1 = x[j];
+        live = true;
+      }
+      if (live) {
+        // pairs i = top .. bot, i.e. p = i - (c0 - 1) from high to low
+        const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
+#pragma unroll
+        for (int p = CW - 1; p >= 0; --p) {
+          const int i = c0 - 1 + p;
+          if (i <= top && i >= bot) {
+            const T2 g = S.rs[q][i - plo];
+            const T h0 = (p >= 1) ? x[p > 0 ? p - 1 : 0] : xlow;
+            x[p] = g.y * h0 + g.x * h1;
+            h1 = g.x * h0 - g.y * h1;
+          }
+        }
+        if (bot == c0 - 1 && q > 0) {
+          S.carry[q - 1][lane] = h1;                 // pending (or final) column c0-1
+        } else if (bot == ilo) {
+#pragma unroll
+          for (int j = 0; j < CW; ++j)
+            if (c0 + j == ilo) x[j] = h1;
+        }
+      }
+      S.topv[q][lane] = x[CW - 1];
+      base += cnt;
+      ++s;
+      code = (s < nsw) ? sw[s] : 0;
+    }
+    __syncthreads();
+  }
+
+  // ---- C. permuted store: column j goes to slot[j] --------------------------
+  if (lane < CW) S.pos[q][lane] = (c0 + lane < n) ? slot[(size_t)mat * n + c0 + lane] : 0;
+  wsync();
+  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
+  if (r < n) {
+#pragma unroll
+    for (int j = 0; j < CW; ++j)
+      if (c0 + j < n) A[r + (size_t)S.pos[q][j] * nm] = x[j];
+  }
+}
+
+} // namespace eigmi

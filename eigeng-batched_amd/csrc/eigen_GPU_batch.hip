@@ -16,6 +16,11 @@
 #include "eig_jacobi.hpp"
 #include "eig_split.hpp"
 #include "eig_uneven.hpp"
+#include "eig_split_large.hpp"
+#include "eig_check.hpp"
+
+#include <stdio.h>
+#include <vector>
 
 #include <mutex>
 #include <map>
@@ -62,6 +67,26 @@ struct SplitLayout {
   }
 };
 
+// Large-order split path (orders beyond the register kernels): workspace
+// regions for a chunk of lc matrices.
+constexpr int LCHUNK_LARGE = 8192;
+constexpr int SPLIT_LARGE_MIN_L = 512;
+struct SplitLargeLayout {
+  size_t wm, de, bw, rot, sw, nsw, status, slot, total;
+  SplitLargeLayout(int lc, int n, int tb)
+  {
+    wm = 0;
+    de = wm + al256((size_t)lc * n * n * tb);
+    bw = de + al256((size_t)2 * n * lc * tb);
+    rot = bw + al256((size_t)lc * n * tb);
+    sw = rot + al256((size_t)lc * rot_cap_of(n) * 2 * tb);
+    nsw = sw + al256((size_t)lc * sw_cap_of(n) * 4);
+    status = nsw + al256((size_t)lc * 4);
+    slot = status + al256((size_t)lc * 4);
+    total = slot + al256((size_t)lc * n * 4);
+  }
+};
+
 bool use_split(int L, int n, int solver)
 {
   const int base = solver & 0xff, flags = solver & ~0xff;
@@ -76,6 +101,16 @@ bool use_split(int L, int n, int solver)
 // beyond.
 bool use_wave_kernel(int n, int tbytes) { return tbytes == 4 ? n <= 128 : n <= 64; }
 
+bool use_split_large(int L, int n, int tbytes, int solver)
+{
+  const int base = solver & 0xff, flags = solver & ~0xff;
+  if (use_wave_kernel(n, tbytes) || n > GL_NMAX) return false;
+  if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2) return false;
+  if (flags & FLAG_FUSED) return false;
+  if (flags & FLAG_SPLIT) return true;
+  return L >= SPLIT_LARGE_MIN_L;
+}
+
 int global_grid(int L) { return L < 1024 ? L : 1024; }
 
 // Workspace: the register kernels keep everything in registers and LDS (a
@@ -88,6 +123,8 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
   if (use_split(L, n, solver)) return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
   if (use_wave_kernel(n, tbytes)) return WS_HEAD;
   if (n > GL_NMAX) return 0;
+  if (use_split_large(L, n, tbytes, solver))
+    return WS_HEAD + SplitLargeLayout(L < LCHUNK_LARGE ? L : LCHUNK_LARGE, n, tbytes).total;
   return WS_HEAD + (size_t)global_grid(L) * (size_t)n * (size_t)n * (size_t)tbytes;
 }
 
@@ -172,15 +209,76 @@ hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver
   const int base = solver & 0xff;
   switch (ntc) {
   case 2:
-    hipLaunchKernelGGL((eig_global_kernel<T, 512>), dim3(grid), dim3(512), 0, s, L, nm, n, a, w, info, base, zws);
+    hipLaunchKernelGGL((eig_global_kernel<T, 512>), dim3(grid), dim3(512), 0, s, L, nm, n, a, w, info, base, zws,
+                       (const int *)nullptr);
     break;
   case 3:
     hipLaunchKernelGGL((eig_global_kernel<T, 1024>), dim3(grid), dim3(1024), 0, s, L, nm, n, a, w, info, base,
-                       zws);
+                       zws, (const int *)nullptr);
     break;
   default:
     hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(grid), dim3(GL_THREADS), 0, s, L, nm, n, a, w,
-                       info, base, zws);
+                       info, base, zws, (const int *)nullptr);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int NCH>
+void launch_apply_large(unsigned grid, int L, int nm, int n, T *a, const T *Wm, const T *bw,
+                        const typename vec2<T>::type *rot, int rcap, const int *sw, int scap, const int *nsw,
+                        const int *st, const int *slot, hipStream_t s)
+{
+  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH>), dim3(grid), dim3(64 * NCH), 0, s, L, nm, n, a, Wm, bw, rot,
+                     rcap, sw, scap, nsw, st, slot);
+}
+
+template <typename T>
+hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
+{
+  typedef typename vec2<T>::type T2;
+  const int lc = L < LCHUNK_LARGE ? L : LCHUNK_LARGE;
+  const SplitLargeLayout lay(lc, n, (int)sizeof(T));
+  char *base = (char *)wk + WS_HEAD;
+  T *Wm = (T *)(base + lay.wm), *de = (T *)(base + lay.de), *bw = (T *)(base + lay.bw);
+  T2 *rot = (T2 *)(base + lay.rot);
+  int *sw = (int *)(base + lay.sw), *nsw = (int *)(base + lay.nsw);
+  int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
+  const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
+  const int sv = solver & 0xff;
+  for (long long c0 = 0; c0 < L; c0 += lc) {
+    const int cur = (int)((L - c0) < lc ? (L - c0) : lc);
+    T *ac = a + (size_t)c0 * nm * n;
+    T *wc = w + (size_t)c0 * n;
+    int *ic = info ? info + c0 : nullptr;
+    const unsigned tg = (unsigned)(cur < 1024 ? cur : 1024);
+    if (n <= 256)
+      hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 256>), dim3(tg), dim3(256), 0, s, cur, nm, n, ac, Wm, de,
+                         lc, bw);
+    else
+      hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 512>), dim3(tg), dim3(256), 0, s, cur, nm, n, ac, Wm, de,
+                         lc, bw);
+    if (n <= 256) {
+      constexpr int NLANE = 16;
+      hipLaunchKernelGGL((eig_ql_lane_kernel<T, 256, NLANE>), dim3((cur + NLANE - 1) / NLANE), dim3(64), 0, s,
+                         cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, sv);
+    } else {
+      constexpr int NLANE = 8;
+      hipLaunchKernelGGL((eig_ql_lane_kernel<T, 512, NLANE>), dim3((cur + NLANE - 1) / NLANE), dim3(64), 0, s,
+                         cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, sv);
+    }
+    const unsigned ag = (unsigned)cur * (unsigned)((n + 63) / 64);
+    if (n <= 128)
+      launch_apply_large<T, 4>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+    else if (n <= 256)
+      launch_apply_large<T, 8>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+    else if (n <= 384)
+      launch_apply_large<T, 12>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+    else
+      launch_apply_large<T, 16>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+    // overflow fallback: the workgroup solver on the flagged matrices, with
+    // the (now free) tridiagonalisation slots as its accumulators
+    hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(tg), dim3(GL_THREADS), 0, s, cur, nm, n, ac, wc,
+                       ic, sv, Wm, (const int *)st);
   }
   return hipGetLastError();
 }
@@ -210,9 +308,11 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, solver, s);
   if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
   if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
-  if (use_wave_kernel(n, (int)sizeof(T))) return launch_wave<T, 128>(L, nm, n, a, w, info, solver, s);
-  if (n <= GL_NMAX) return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
-  return hipErrorNotSupported;
+  if (use_wave_kernel(n, (int)sizeof(T)))
+    return launch_wave<T, (sizeof(T) == 4 ? 128 : 64)>(L, nm, n, a, w, info, solver, s);
+  if (n > GL_NMAX) return hipErrorNotSupported;
+  if (use_split_large(L, n, (int)sizeof(T), solver)) return launch_split_large<T>(L, nm, n, a, w, info, solver, wk, s);
+  return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
 }
 
 // ---- uneven batches ---------------------------------------------------------
@@ -315,6 +415,41 @@ hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, con
   return hipGetLastError();
 }
 
+// ---- accuracy check -----------------------------------------------------------
+
+template <typename T>
+hipError_t run_check(int L, int nm, int n, int m, const T *a, const T *w, const T *z, double *out, int into_a,
+                     hipStream_t s)
+{
+  if (L < 1 || n < 1 || nm < n || m < 1 || m > n || n > GL_NMAX || !a || !w || !z) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)(L < 4096 ? L : 4096);
+  hipLaunchKernelGGL((eig_check_kernel<T>), dim3(grid), dim3(CK_THREADS), 0, s, L, nm, n, m, a, w, z, out, into_a);
+  return hipGetLastError();
+}
+
+// The reference's eigen_GPU_check_RUN (eigen_GPU_check.cu:143-182): report
+// written over each a_k, then read back and the failing matrices printed.
+template <typename T>
+void check_entry(int L, int nm, int n, int m, T *a, T *w, T *z, hipStream_t s)
+{
+  if (run_check<T>(L, nm, n, m, a, w, z, nullptr, 1, s) != hipSuccess) return;
+  std::vector<float> rep((size_t)L * 4);
+  const size_t pitch = (size_t)nm * n * sizeof(T);
+  if (hipMemcpy2DAsync(rep.data(), 4 * sizeof(float), a, pitch, 4 * sizeof(float), L, hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return;
+  if (hipStreamSynchronize(s) != hipSuccess) return;
+  for (int id = 0; id < L; ++id) {
+    const float *ans = &rep[(size_t)id * 4];
+    if (ans[0] != 0.0f)
+      printf("[%06d] Accuracy error in the relative resisdual (||Ax-dx||_F=%le/%le/%le)\n", id, (double)ans[0],
+             (double)ans[2], (double)ans[3]);
+    if (ans[1] != 0.0f)
+      printf("[%06d] Orthonormality error (||ZZ-I||_F=%le/%le/%le)\n", id, (double)ans[1], (double)ans[2],
+             (double)ans[3]);
+  }
+}
+
 template <typename T>
 void abi_entry(int L, int nm, int n, int m, T *a, T *w, T *wk, hipStream_t s)
 {
@@ -380,6 +515,32 @@ int eigen_GPU_batch_uneven_DP(int L, int nmax, const int *n, const long long *ao
                               double *a, double *w, void *wk, hipStream_t stream, int solver, int *info)
 {
   return (int)run_uneven<double>(L, nmax, n, aoff, woff, a, w, wk, stream, solver, info);
+}
+
+void eigen_GPU_check_DP(const int L, const int nm, const int n, const int m, double *a_, double *w_, double *z_,
+                        const gpuStream_t stream)
+{
+  check_entry<double>(L, nm, n, m, a_, w_, z_, stream);
+}
+
+void eigen_GPU_check_FP(const int L, const int nm, const int n, const int m, float *a_, float *w_, float *z_,
+                        const gpuStream_t stream)
+{
+  check_entry<float>(L, nm, n, m, a_, w_, z_, stream);
+}
+
+int eigen_GPU_check_x_DP(int L, int nm, int n, int m, const double *a, const double *w, const double *z,
+                         double *out, hipStream_t stream)
+{
+  if (!out) return (int)hipErrorInvalidValue;
+  return (int)run_check<double>(L, nm, n, m, a, w, z, out, 0, stream);
+}
+
+int eigen_GPU_check_x_FP(int L, int nm, int n, int m, const float *a, const float *w, const float *z,
+                         double *out, hipStream_t stream)
+{
+  if (!out) return (int)hipErrorInvalidValue;
+  return (int)run_check<float>(L, nm, n, m, a, w, z, out, 0, stream);
 }
 
 const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.1"; }

@@ -57,6 +57,14 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.argtypes = [ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, vp]
             f.restype = ci
+        for name in ("eigen_GPU_check_x_FP", "eigen_GPU_check_x_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp]
+            f.restype = ci
+        for name in ("eigen_GPU_check_FP", "eigen_GPU_check_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp]
+            f.restype = None
         L.eigen_GPU_batch_uneven_bufsize.argtypes = [ci, ci, ci, ci]
         L.eigen_GPU_batch_uneven_bufsize.restype = sz
         L.eigen_GPU_batch_bufsize.argtypes = [ci, ci, ci, ci, ci, ci]
@@ -204,6 +212,21 @@ def eigen_batch_uneven(a, nv, nmax: int, w=None, aoff=None, woff=None, solver: i
     if rc != 0:
         raise RuntimeError(f"eigen_GPU_batch_uneven_{'FP' if tb == 4 else 'DP'} failed: hipError {rc}")
     return a, w, info
+
+
+def check(a_full, w, z, m: int | None = None, stream=None):
+    """GPU accuracy check (eigen_GPU_check_x_*): a_full (L, n, nm) full
+    symmetric inputs, w (L, n), z (L, n, nm) eigenvectors.  Returns a (L, 4)
+    float64 device tensor: err1, err2, TOL, fail bits (1 residual, 2 orth)."""
+    import torch
+    L, n, nm = a_full.shape
+    m = n if m is None else m
+    out = torch.empty((L, 4), dtype=torch.float64, device=a_full.device)
+    f = lib().eigen_GPU_check_x_FP if a_full.element_size() == 4 else lib().eigen_GPU_check_x_DP
+    rc = f(L, nm, n, m, a_full.data_ptr(), w.data_ptr(), z.data_ptr(), out.data_ptr(), _stream_ptr(stream))
+    if rc != 0:
+        raise RuntimeError(f"eigen_GPU_check_x failed: hipError {rc}")
+    return out
 
 
 def version() -> str:

@@ -118,6 +118,30 @@ int eigen_GPU_batch_uneven_DP(int L, int nmax, const int *n, const long long *ao
                               const long long *woff, double *a, double *w, void *wk,
                               hipStream_t stream, int solver, int *info);
 
+/*
+ * Accuracy check of a batch of eigenpairs, in double, on the GPU.
+ * a: the full symmetric input matrices (leading dimension nm), w: eigenvalues,
+ * z: eigenvectors (leading dimension nm), m: number of eigenpairs checked.
+ * err1 = ||A Z - Z D||_F, err2 = ||Z^T Z - I||_F, TOL = eps (2*512-1 | 2*16-1)
+ * sqrt(n+16); a matrix fails if err1 > max(|w_1|, |w_n|) TOL or err2 > n TOL.
+ */
+/* Replaces eigen_GPU_check_DP / _FP (reference eigen_GPU_check.h:20-23,
+ * eigen_GPU_check.cu:186-198): the report (err1/max|w| or 0, err2 or 0, TOL,
+ * eps, as floats) is written over the first four floats of each a_k, then read
+ * back (the call synchronizes `stream`) and every failure printed to stdout. */
+void eigen_GPU_check_DP(const int L, const int nm, const int n, const int m,
+                        double *a_, double *w_, double *z_, const gpuStream_t stream);
+void eigen_GPU_check_FP(const int L, const int nm, const int n, const int m,
+                        float *a_, float *w_, float *z_, const gpuStream_t stream);
+/* Extension: asynchronous, inputs untouched; out (device double[4 L]) gets
+ * (err1, err2, TOL, fail bits: 1 residual, 2 orthogonality) per matrix.
+ * Returns hipSuccess, hipErrorInvalidValue (n > 512 or bad sizes) or the
+ * launch error. */
+int eigen_GPU_check_x_DP(int L, int nm, int n, int m, const double *a, const double *w,
+                         const double *z, double *out, hipStream_t stream);
+int eigen_GPU_check_x_FP(int L, int nm, int n, int m, const float *a, const float *w,
+                         const float *z, double *out, hipStream_t stream);
+
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);
 

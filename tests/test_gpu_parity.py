@@ -149,6 +149,43 @@ def test_large_n_vs_oracle(n, dtype):
     check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
 
 
+LARGE_SPLIT_SIZES = [65, 96, 129, 160, 200, 255, 256, 257, 300, 384, 512]
+
+
+@pytest.mark.parametrize("path", ["split", "split-fallback"])
+@pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", LARGE_SPLIT_SIZES)
+def test_large_split_vs_oracle(n, dtype, solver, path):
+    """Orders beyond the register kernels through the split path (workgroup
+    tridiagonalisation, lane-per-matrix QL with rotation log, row-block apply)
+    and through its overflow fallback (log capacity forced to 1)."""
+    if dtype == np.float32 and n <= 128:
+        pytest.skip("FP32 n <= 128 is served by the register kernels")
+    print(f"[parity] large {path} solver={solver} n={n} dtype={np.dtype(dtype).name}", flush=True)
+    nm = n + (n & 1)
+    L = 7 if n <= 256 else 3
+    a0 = wl.sym_rand(L, n, nm, dtype=dtype, seed=7000 + n)
+    zo, wo, info_o = oracle.eig_batch(np.nan_to_num(a0), oracle.TQL2 if solver == eg.TQL2 else oracle.IMTQL2)
+    assert info_o.max() == 0
+    z, w, info = run_gpu(a0, solver=solver, path=path)
+    assert info.max() == 0 and info.min() == 0
+    if nm > n:
+        assert np.isnan(z[:, :, n:]).all()
+    check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("dtype", [np.float64])
+def test_large_split_letkf_and_qdq(dtype):
+    """n = 256 clustered spectra (LETKF-like, and pairs split by 1e-8) through
+    the large split path."""
+    for a0 in (wl.letkf_y(6, k=256, p=400, rho=1.05, sigma=0.1, dtype=dtype, seed=5)[0],):
+        zo, wo, _ = oracle.eig_batch(a0, oracle.IMTQL2)
+        z, w, info = run_gpu(a0, path="split")
+        assert info.max() == 0
+        check_batch(a0, z, w, dtype, wo, zo)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [5, 32, 64, 100, 200])
 def test_tql2_variant(n, dtype):
