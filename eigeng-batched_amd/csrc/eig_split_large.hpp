@@ -13,8 +13,8 @@
 //                          Q = H_{n-1} ... H_1 formed in registers, then every
 //                          logged sweep replayed on them, then the permuted store.
 //
-// Apply kernel layout: lane = row of the block, wave q = 32-column chunk
-// [32q, 32q+31] held in registers.  Rotations act on columns, so rows are
+// Apply kernel layout: lane = row of the block, wave q = CW-column chunk
+// [CW q, CW q + CW-1] held in registers (the description below uses CW = 32).  Rotations act on columns, so rows are
 // independent, but one sweep chains through the columns from the top of its
 // window down: wave q handles the pairs i in [32q-1, 32q+30] (pair i finalises
 // column i+1, i.e. the wave's own columns).  Pair 32q-1 needs the top column of
@@ -31,8 +31,6 @@
 
 namespace eigmi {
 
-constexpr int AL_CW = 32;  // columns per wave in the apply kernel
-
 // ---- blocked Householder tridiagonalisation ---------------------------------
 // The reduction of column c needs A11 = A(0:c-1, 0:c-1) with the rank-2
 // updates of all earlier columns applied.  In panels of NBP columns those
@@ -44,11 +42,12 @@ constexpr int AL_CW = 32;  // columns per wave in the apply kernel
 // over A11 instead of a read pass plus a read-modify-write pass; the
 // Householder convention (anorm, beta, u_c stored in A(0:c-1, c), d, e) is the
 // unblocked one's.
-template <typename T, int NBP, int NMAXT>
+template <typename T, int NBP, int NMAXT, int RG>
 struct alignas(16) TriBlkLds {
   alignas(16) T U[NBP][NMAXT];
   alignas(16) T V[NBP][NMAXT];
   alignas(16) T u[NMAXT];
+  alignas(16) T pp[RG][NMAXT];                   // per-column-group partial products
   alignas(16) T red[16][2 * NBP + 2];
   alignas(16) T d[NMAXT];
   alignas(16) T e[NMAXT];
@@ -57,8 +56,8 @@ struct alignas(16) TriBlkLds {
 
 // Sum over the workgroup of K per-thread values (K <= 2 NBP + 2); every thread
 // gets the K sums.
-template <int NT, int K, int NBP, typename T, int NMAXT>
-__device__ __forceinline__ void block_sums(T (&x)[K], int cnt, TriBlkLds<T, NBP, NMAXT> &S)
+template <int NT, int K, typename S_t, typename T>
+__device__ __forceinline__ void block_sums(T (&x)[K], int cnt, S_t &S)
 {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -81,12 +80,17 @@ __device__ __forceinline__ void block_sums(T (&x)[K], int cnt, TriBlkLds<T, NBP,
   __syncthreads();
 }
 
-template <typename T, int NT, int NBP, int NMAXT>
-__device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, NBP, NMAXT> &S)
+// Thread t works on rows t % RT and t % RT + RT (RT = NT / RG) and, in the two
+// passes over the trailing matrix, on the columns j = g (mod RG), g = t / RT.
+template <typename T, int NT, int NBP, int NMAXT, int RG>
+__device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, NBP, NMAXT, RG> &S)
 {
-  static_assert(2 * NT >= NMAXT, "two rows per thread cover the matrix");
+  constexpr int RT = NT / RG;
+  static_assert(2 * RT >= NMAXT, "two rows per thread cover the matrix");
   const int t = threadIdx.x;
-  const int r0 = t, r1 = t + NT;
+  const int g = t / RT;
+  const int r0 = t - g * RT, r1 = r0 + RT;
+  const bool lead = (g == 0);                        // the group that owns per-row state
   // symmetrise (lower <- upper)
   for (int c = 0; c < n; ++c)
     for (int r = c + 1 + t; r < n; r += NT) A[r + (size_t)c * nm] = A[c + (size_t)r * nm];
@@ -102,7 +106,7 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
       const int c = ctop - k;
       // 1. column c of the updated matrix (rows < c) and its diagonal
       T uc0 = T(0), uc1 = T(0);
-      {
+      if (lead) {
         T ukc[NBP], vkc[NBP];
 #pragma unroll
         for (int kk = 0; kk < NBP; ++kk) { ukc[kk] = S.U[kk][c]; vkc[kk] = S.V[kk][c]; }
@@ -120,11 +124,11 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
             if (kk < k) acc = fma_(S.U[kk][r1], vkc[kk], fma_(S.V[kk][r1], ukc[kk], acc));
           if (r1 < c) uc1 = acc; else S.d[c] = acc;
         }
+        if (r0 < c) S.u[r0] = uc0;
+        if (r1 < c) S.u[r1] = uc1;
       }
-      if (r0 < c) S.u[r0] = uc0;
-      if (r1 < c) S.u[r1] = uc1;
       T sg[1] = {uc0 * uc0 + uc1 * uc1};
-      block_sums<NT, 1, NBP>(sg, 1, S);
+      block_sums<NT, 1>(sg, 1, S);
       const T ulast = S.u[c - 1];
       const T anorm = -sign_(sqrt_(sg[0]), ulast);
       const T ul = ulast - anorm;
@@ -132,8 +136,10 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
       if (r0 == c - 1) uc0 = ul;
       if (r1 == c - 1) uc1 = ul;
       // u_c is stored in A(0:c-1, c) (with u(c-1) = ul), as in the unblocked form
-      if (r0 < c) A[r0 + (size_t)c * nm] = uc0;
-      if (r1 < c) A[r1 + (size_t)c * nm] = uc1;
+      if (lead) {
+        if (r0 < c) A[r0 + (size_t)c * nm] = uc0;
+        if (r1 < c) A[r1 + (size_t)c * nm] = uc1;
+      }
       __syncthreads();
       if (t == 0) {
         S.u[c - 1] = ul;
@@ -145,44 +151,65 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
       T dd[2 * NBP];
 #pragma unroll
       for (int kk = 0; kk < 2 * NBP; ++kk) dd[kk] = T(0);
+      if (lead) {
 #pragma unroll
-      for (int kk = 0; kk < NBP; ++kk) {
-        if (kk < k) {
-          if (r0 < c) { dd[2 * kk] = S.V[kk][r0] * uc0; dd[2 * kk + 1] = S.U[kk][r0] * uc0; }
-          if (r1 < c) {
-            dd[2 * kk] = fma_(S.V[kk][r1], uc1, dd[2 * kk]);
-            dd[2 * kk + 1] = fma_(S.U[kk][r1], uc1, dd[2 * kk + 1]);
+        for (int kk = 0; kk < NBP; ++kk) {
+          if (kk < k) {
+            if (r0 < c) { dd[2 * kk] = S.V[kk][r0] * uc0; dd[2 * kk + 1] = S.U[kk][r0] * uc0; }
+            if (r1 < c) {
+              dd[2 * kk] = fma_(S.V[kk][r1], uc1, dd[2 * kk]);
+              dd[2 * kk + 1] = fma_(S.U[kk][r1], uc1, dd[2 * kk + 1]);
+            }
           }
         }
       }
-      block_sums<NT, 2 * NBP, NBP>(dd, 2 * k, S);
+      block_sums<NT, 2 * NBP>(dd, 2 * k, S);
+      // A(0:c-1, 0:c-1) u: column group g sums over j = g (mod RG)
       T p0 = T(0), p1 = T(0);
       if (r0 < c) {
-        for (int j = 0; j < c; ++j) {
+        for (int j = g; j < c; j += RG) {
           const T uj = S.u[j];
           p0 = fma_(A[r0 + (size_t)j * nm], uj, p0);
           if (r1 < c) p1 = fma_(A[r1 + (size_t)j * nm], uj, p1);
         }
       }
+      if (RG > 1) {
+        S.pp[g][r0] = p0;
+        if (r1 < NMAXT) S.pp[g][r1] = p1;
+        __syncthreads();
+        p0 = T(0);
+        p1 = T(0);
+        if (lead) {
 #pragma unroll
-      for (int kk = 0; kk < NBP; ++kk) {
-        if (kk < k) {
-          if (r0 < c) p0 = fma_(S.U[kk][r0], dd[2 * kk], fma_(S.V[kk][r0], dd[2 * kk + 1], p0));
-          if (r1 < c) p1 = fma_(S.U[kk][r1], dd[2 * kk], fma_(S.V[kk][r1], dd[2 * kk + 1], p1));
+          for (int gg = 0; gg < RG; ++gg) {
+            p0 += S.pp[gg][r0];
+            if (r1 < NMAXT) p1 += S.pp[gg][r1];
+          }
         }
       }
-      p0 = (r0 < c) ? p0 * beta : T(0);
-      p1 = (r1 < c) ? p1 * beta : T(0);
+      if (lead) {
+#pragma unroll
+        for (int kk = 0; kk < NBP; ++kk) {
+          if (kk < k) {
+            if (r0 < c) p0 = fma_(S.U[kk][r0], dd[2 * kk], fma_(S.V[kk][r0], dd[2 * kk + 1], p0));
+            if (r1 < c) p1 = fma_(S.U[kk][r1], dd[2 * kk], fma_(S.V[kk][r1], dd[2 * kk + 1], p1));
+          }
+        }
+      }
+      p0 = (lead && r0 < c) ? p0 * beta : T(0);
+      p1 = (lead && r1 < c) ? p1 * beta : T(0);
       T al[1] = {p0 * uc0 + p1 * uc1};
-      block_sums<NT, 1, NBP>(al, 1, S);
+      block_sums<NT, 1>(al, 1, S);
       const T alpha = al[0] * (beta * T(0.5));
-      if (r0 < NMAXT) { S.U[k][r0] = (r0 < c) ? uc0 : T(0); S.V[k][r0] = (r0 < c) ? fma_(alpha, uc0, p0) : T(0); }
-      if (r1 < NMAXT) { S.U[k][r1] = (r1 < c) ? uc1 : T(0); S.V[k][r1] = (r1 < c) ? fma_(alpha, uc1, p1) : T(0); }
+      if (lead) {
+        if (r0 < NMAXT) { S.U[k][r0] = (r0 < c) ? uc0 : T(0); S.V[k][r0] = (r0 < c) ? fma_(alpha, uc0, p0) : T(0); }
+        if (r1 < NMAXT) { S.U[k][r1] = (r1 < c) ? uc1 : T(0); S.V[k][r1] = (r1 < c) ? fma_(alpha, uc1, p1) : T(0); }
+      }
       __syncthreads();
     }
     // 3. panel update of the trailing block A(0:clow-1, 0:clow-1)
     const int clow = ctop - nbk + 1;
-    if (clow > 1 || (clow == 1)) {
+    {
       T ur0[NBP], vr0[NBP], ur1[NBP], vr1[NBP];
 #pragma unroll
       for (int kk = 0; kk < NBP; ++kk) {
@@ -192,7 +219,7 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
         vr1[kk] = (r1 < clow && kk < nbk) ? S.V[kk][r1] : T(0);
       }
       if (r0 < clow) {
-        for (int j = 0; j < clow; ++j) {
+        for (int j = g; j < clow; j += RG) {
           T a0 = A[r0 + (size_t)j * nm];
           T a1 = (r1 < clow) ? A[r1 + (size_t)j * nm] : T(0);
 #pragma unroll
@@ -219,12 +246,12 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
   __syncthreads();
 }
 
-template <typename T, int NT, int NBP, int NMAXT>
+template <typename T, int NT, int NBP, int NMAXT, int RG>
 __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n, const T *__restrict__ a,
                                                            T *__restrict__ W, T *__restrict__ de, int ld,
                                                            T *__restrict__ bws)
 {
-  __shared__ TriBlkLds<T, NBP, NMAXT> S;
+  __shared__ TriBlkLds<T, NBP, NMAXT, RG> S;
   const int t = threadIdx.x;
   for (long long mat = blockIdx.x; mat < L; mat += gridDim.x) {
     const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
@@ -232,7 +259,7 @@ __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n,
     for (int c = 0; c < n; ++c)
       for (int r = t; r <= c; r += NT) Wm[r + (size_t)c * n] = A[r + (size_t)c * nm];
     __syncthreads();
-    blocked_tridiag<T, NT, NBP, NMAXT>(Wm, n, n, S);
+    blocked_tridiag<T, NT, NBP, NMAXT, RG>(Wm, n, n, S);
     for (int i = t; i < n; i += NT) {
       de[(size_t)i * ld + mat] = S.d[i];
       de[(size_t)(n + i) * ld + mat] = S.e[i];
@@ -242,25 +269,31 @@ __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n,
   }
 }
 
-template <typename T, int NCH>
+template <typename T, int NCH, int CW>
 struct alignas(16) ApplyLargeLds {
   alignas(16) T red[2][NCH][64];                 // per-wave partial dot products (phase A)
   alignas(16) T topv[NCH][64];                   // wave q's top column after its last sweep
   alignas(16) T carry[NCH][64];                  // pending chain value handed to wave q
-  alignas(16) T us[NCH][AL_CW];                  // reflector entries of wave q's columns
-  alignas(16) typename vec2<T>::type rs[NCH][AL_CW];  // rotations of wave q's current sweep
-  alignas(16) int pos[NCH][AL_CW];
+  alignas(16) T us[NCH][CW];                     // reflector entries of wave q's columns
+  alignas(16) typename vec2<T>::type rs[NCH][CW];  // rotations of wave q's current sweep
+  alignas(16) int pos[NCH][CW];
+  alignas(16) T beta[GL_NMAX];
 };
 
-template <typename T, int NCH>
-__global__ __launch_bounds__(64 * NCH) void eig_apply_large_kernel(
+// CW columns per wave (32, or 16 for FP64 so that the 16 doubles of a row
+// chunk and the chain fit next to each other in registers)
+// FP64 is held to 128 VGPRs (four waves per SIMD: two 8-wave workgroups per
+// CU); FP32 keeps the registers its branch-free chain wants
+template <typename T> struct ApWaves { static constexpr int value = sizeof(T) == 8 ? 4 : 1; };
+
+template <typename T, int NCH, int CW>
+__global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWaves<T>::value))) void eig_apply_large_kernel(
     int L, int nm, int n, T *__restrict__ a, const T *__restrict__ W, const T *__restrict__ bws,
     const typename vec2<T>::type *__restrict__ rotlog, int rot_cap, const int *__restrict__ swlog, int sw_cap,
     const int *__restrict__ nsw_in, const int *__restrict__ status, const int *__restrict__ slot)
 {
   typedef typename vec2<T>::type T2;
-  constexpr int CW = AL_CW;
-  __shared__ ApplyLargeLds<T, NCH> S;
+  __shared__ ApplyLargeLds<T, NCH, CW> S;
   const int nrb = (n + 63) >> 6;
   const long long mat = blockIdx.x / nrb;
   const int rb = blockIdx.x - (int)(mat * nrb);
@@ -278,10 +311,15 @@ __global__ __launch_bounds__(64 * NCH) void eig_apply_large_kernel(
 
   // ---- A. rows of Q = H_{n-1} ... H_1: x <- x H_c, c = n-1 down to 1 --------
   // (the right-multiplication of wave_front, split over the column chunks)
+  for (int i = threadIdx.x; i < n; i += 64 * NCH) S.beta[i] = beta[i];
+  // reflector entries are prefetched two reflectors ahead
   T unext = (lane < CW && c0 + lane < n - 1) ? Wm[c0 + lane + (size_t)(n - 1) * n] : T(0);
+  T unext2 = (lane < CW && n >= 3 && c0 + lane < n - 2) ? Wm[c0 + lane + (size_t)(n - 2) * n] : T(0);
+  __syncthreads();
   for (int c = n - 1; c >= 1; --c) {
     const T ucur = (c0 + lane < c) ? unext : T(0);
-    if (c > 1 && lane < CW) unext = (c0 + lane < c - 1) ? Wm[c0 + lane + (size_t)(c - 1) * n] : T(0);
+    unext = unext2;
+    if (c > 2 && lane < CW) unext2 = (c0 + lane < c - 2) ? Wm[c0 + lane + (size_t)(c - 2) * n] : T(0);
     T part = T(0);
     if (c0 < c) {
       if (lane < CW) S.us[q][lane] = ucur;
@@ -294,7 +332,7 @@ __global__ __launch_bounds__(64 * NCH) void eig_apply_large_kernel(
     T tsum = T(0);
 #pragma unroll
     for (int w2 = 0; w2 < NCH; ++w2) tsum += S.red[c & 1][w2][lane];
-    tsum *= beta[c];
+    tsum *= S.beta[c];
     if (c0 < c) {
 #pragma unroll
       for (int j = 0; j < CW; ++j) x[j] = fma_(tsum, S.us[q][j], x[j]);
@@ -309,8 +347,19 @@ __global__ __launch_bounds__(64 * NCH) void eig_apply_large_kernel(
   // this wave's pairs: plo .. phi (pair i finalises column i+1)
   const int phi = c0 + CW - 2;
   const int plo = (c0 - 1 > 0) ? c0 - 1 : 0;
+  // Software pipeline: the rotations of the wave's next sweep are loaded
+  // during the current one (lane j: pair c0-1+j, identity outside the sweep's
+  // window), the sweep code one sweep further ahead.
+  auto fetch = [&](int cd, int bs) -> T2 {
+    const int m1 = sw_m(cd), ilo1 = sw_ilo(cd);
+    const int i = c0 - 1 + lane;
+    const bool in = lane < CW && i >= plo && i <= phi && i >= ilo1 && i <= m1 - 1;
+    return in ? lg[bs + (m1 - 1 - i)] : T2{T(1), T(0)};
+  };
   int base = 0;                                      // log offset of the wave's next sweep
   int code = (nsw > 0) ? sw[0] : 0;
+  int code1 = (nsw > 1) ? sw[1] : 0;
+  T2 pre = (nsw > 0) ? fetch(code, 0) : T2{T(1), T(0)};
   S.topv[q][lane] = x[CW - 1];
   __syncthreads();
   const int nsteps = (nsw > 0) ? 2 * (nsw - 1) + NCH : 0;
@@ -323,53 +372,77 @@ __global__ __launch_bounds__(64 * NCH) void eig_apply_large_kernel(
       const int top = (m - 1 < phi) ? m - 1 : phi;
       const int bot = (ilo > plo) ? ilo : plo;
       const bool enters = (m - 1 > phi) && (ilo <= phi + 1);
-      // rotations of this wave's pairs (pair i at log position base + m-1-i)
-      if (top >= bot && lane < CW) {
-        const int i = top - lane;
-        if (i >= bot) S.rs[q][i - plo] = lg[base + (m - 1 - i)];
-      }
+      if (lane < CW) S.rs[q][lane] = pre;
+      const int code2 = (s + 2 < nsw) ? sw[s + 2] : 0;
+      pre = (s + 1 < nsw) ? fetch(code1, base + cnt) : T2{T(1), T(0)};
       wsync();
-      T h1 = T(0);
-      bool live = false;
-      if (enters) {
-        h1 = S.carry[q][lane];
-        live = true;
-        if (ilo == phi + 1) {                        // the chain ended on the pair above
-          x[CW - 1] = h1;
-          live = false;
-        }
-      } else if (m - 1 >= plo && m - 1 <= phi) {
-        // the window's top column m is this wave's column m - c0
+      if constexpr (sizeof(T) == 4) {
+        if (enters || top >= bot) {
+          // Branch-free chain over all pairs of the chunk: outside [bot, top]
+          // the rotation is the identity, which leaves every column as it is
+          // while h1 carries the original value of the column above (so the
+          // window starts with h1 = x(m), and x(ilo) receives the pending value).
+          T h1 = enters ? S.carry[q][lane] : x[CW - 1];
+          const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
 #pragma unroll
-        for (int j = 0; j < CW; ++j)
-          if (c0 + j == m) h1 = x[j];
-        live = true;
-      }
-      if (live) {
-        // pairs i = top .. bot, i.e. p = i - (c0 - 1) from high to low
-        const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
-#pragma unroll
-        for (int p = CW - 1; p >= 0; --p) {
-          const int i = c0 - 1 + p;
-          if (i <= top && i >= bot) {
-            const T2 g = S.rs[q][i - plo];
-            const T h0 = (p >= 1) ? x[p > 0 ? p - 1 : 0] : xlow;
+          for (int p = CW - 1; p >= 1; --p) {
+            const T2 g = S.rs[q][p];
+            const T h0 = x[p - 1];
             x[p] = g.y * h0 + g.x * h1;
             h1 = g.x * h0 - g.y * h1;
           }
+          if (q > 0) {
+            const T2 g = S.rs[q][0];
+            x[0] = g.y * xlow + g.x * h1;
+            S.carry[q - 1][lane] = g.x * xlow - g.y * h1;  // pending (or final) column c0-1
+          } else {
+            x[0] = h1;                                      // column 0 of the matrix
+          }
         }
-        if (bot == c0 - 1 && q > 0) {
-          S.carry[q - 1][lane] = h1;                 // pending (or final) column c0-1
-        } else if (bot == ilo) {
+      } else {
+        // FP64: only the pairs inside the window (the branch-free form needs
+        // more registers than the doubles of the chunk leave free)
+        T h1 = T(0);
+        bool live = false;
+        if (enters) {
+          h1 = S.carry[q][lane];
+          live = true;
+          if (ilo == phi + 1) {                      // the chain ended on the pair above
+            x[CW - 1] = h1;
+            live = false;
+          }
+        } else if (m - 1 >= plo && m - 1 <= phi) {
 #pragma unroll
           for (int j = 0; j < CW; ++j)
-            if (c0 + j == ilo) x[j] = h1;
+            if (c0 + j == m) h1 = x[j];
+          live = true;
+        }
+        if (live) {
+          const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
+#pragma unroll
+          for (int p = CW - 1; p >= 0; --p) {
+            const int i = c0 - 1 + p;
+            if (i <= top && i >= bot) {
+              const T2 g = S.rs[q][p];
+              const T h0 = (p >= 1) ? x[p > 0 ? p - 1 : 0] : xlow;
+              x[p] = g.y * h0 + g.x * h1;
+              h1 = g.x * h0 - g.y * h1;
+            }
+          }
+          if (bot == c0 - 1 && q > 0) {
+            S.carry[q - 1][lane] = h1;               // pending (or final) column c0-1
+          } else if (bot == ilo) {
+#pragma unroll
+            for (int j = 0; j < CW; ++j)
+              if (c0 + j == ilo) x[j] = h1;
+          }
         }
       }
       S.topv[q][lane] = x[CW - 1];
       base += cnt;
       ++s;
-      code = (s < nsw) ? sw[s] : 0;
+      code = code1;
+      code1 = code2;
     }
     __syncthreads();
   }

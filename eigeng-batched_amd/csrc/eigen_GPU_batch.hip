@@ -198,6 +198,10 @@ hipError_t launch_jacobi(int L, int nm, int n, T *a, T *w, int *info, hipStream_
 // defaults.  Fewer, wider workgroups keep the matrices being worked on (and
 // their accumulators) inside the on-chip caches.
 constexpr int GL_NT_SHIFT = 16, GL_GRID_SHIFT = 20;
+// large split path, tridiagonalisation shape (bits 24-25): 1: 256 threads,
+// 2: 512 threads in 2 column groups, 3: 1024 threads in 4 column groups
+// (2 for n > 256); 0 picks the default
+constexpr int TRI_CFG_SHIFT = 24;
 
 template <typename T>
 hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
@@ -223,13 +227,13 @@ hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver
   return hipGetLastError();
 }
 
-template <typename T, int NCH>
+template <typename T, int NCH, int CW>
 void launch_apply_large(unsigned grid, int L, int nm, int n, T *a, const T *Wm, const T *bw,
                         const typename vec2<T>::type *rot, int rcap, const int *sw, int scap, const int *nsw,
                         const int *st, const int *slot, hipStream_t s)
 {
-  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH>), dim3(grid), dim3(64 * NCH), 0, s, L, nm, n, a, Wm, bw, rot,
-                     rcap, sw, scap, nsw, st, slot);
+  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH, CW>), dim3(grid), dim3(64 * NCH), 0, s, L, nm, n, a, Wm, bw,
+                     rot, rcap, sw, scap, nsw, st, slot);
 }
 
 template <typename T>
@@ -245,19 +249,41 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
   int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
   const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
   const int sv = solver & 0xff;
+  const int ph = (solver & FLAG_PH_ALL) ? (solver & FLAG_PH_ALL) : FLAG_PH_ALL;
+  if (ph != FLAG_PH_ALL && L > lc) return hipErrorInvalidValue;
   for (long long c0 = 0; c0 < L; c0 += lc) {
     const int cur = (int)((L - c0) < lc ? (L - c0) : lc);
     T *ac = a + (size_t)c0 * nm * n;
     T *wc = w + (size_t)c0 * n;
     int *ic = info ? info + c0 : nullptr;
     const unsigned tg = (unsigned)(cur < 1024 ? cur : 1024);
-    if (n <= 256)
-      hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 256>), dim3(tg), dim3(256), 0, s, cur, nm, n, ac, Wm, de,
-                         lc, bw);
-    else
-      hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 512>), dim3(tg), dim3(256), 0, s, cur, nm, n, ac, Wm, de,
-                         lc, bw);
-    if (n <= 256) {
+    if (ph & FLAG_PH_TRI) {
+      // tridiagonalisation shape: workgroup size / column groups and grid
+      const int tc = (solver >> TRI_CFG_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
+      const int cfg = tc ? tc : (n <= 256 ? 2 : 1);   // measured best at n = 256 FP64: 512 threads
+      unsigned grid = tg;
+      if (gc) grid = (unsigned)(128 * gc) < grid ? (unsigned)(128 * gc) : grid;
+      if (n <= 256) {
+        if (cfg == 3)
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, 8, 256, 4>), dim3(grid), dim3(1024), 0, s, cur, nm, n,
+                             ac, Wm, de, lc, bw);
+        else if (cfg == 2)
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 512, 8, 256, 2>), dim3(grid), dim3(512), 0, s, cur, nm, n, ac,
+                             Wm, de, lc, bw);
+        else
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 256, 1>), dim3(grid), dim3(256), 0, s, cur, nm, n, ac,
+                             Wm, de, lc, bw);
+      } else {
+        if (cfg == 3)
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, 8, 512, 2>), dim3(grid), dim3(1024), 0, s, cur, nm, n,
+                             ac, Wm, de, lc, bw);
+        else
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 512, 1>), dim3(grid), dim3(256), 0, s, cur, nm, n, ac,
+                             Wm, de, lc, bw);
+      }
+    }
+    if (!(ph & FLAG_PH_QL)) {
+    } else if (n <= 256) {
       constexpr int NLANE = 16;
       hipLaunchKernelGGL((eig_ql_lane_kernel<T, 256, NLANE>), dim3((cur + NLANE - 1) / NLANE), dim3(64), 0, s,
                          cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, sv);
@@ -266,15 +292,17 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
       hipLaunchKernelGGL((eig_ql_lane_kernel<T, 512, NLANE>), dim3((cur + NLANE - 1) / NLANE), dim3(64), 0, s,
                          cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, sv);
     }
+    if (!(ph & FLAG_PH_APPLY)) continue;
     const unsigned ag = (unsigned)cur * (unsigned)((n + 63) / 64);
+    constexpr int CWS = 32;  // chunk width
     if (n <= 128)
-      launch_apply_large<T, 4>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 128 / CWS, CWS>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else if (n <= 256)
-      launch_apply_large<T, 8>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 256 / CWS, CWS>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else if (n <= 384)
-      launch_apply_large<T, 12>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 12, 32>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else
-      launch_apply_large<T, 16>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 16, 32>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     // overflow fallback: the workgroup solver on the flagged matrices, with
     // the (now free) tridiagonalisation slots as its accumulators
     hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(tg), dim3(GL_THREADS), 0, s, cur, nm, n, ac, wc,
@@ -290,7 +318,8 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   const int base = solver & 0xff;
   if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2 && base != SOLVER_JACOBI) return hipErrorNotSupported;
   if (wk == nullptr) return hipErrorInvalidValue;
-  if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver)) return hipErrorInvalidValue;
+  if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver) && !use_split_large(L, n, (int)sizeof(T), solver))
+    return hipErrorInvalidValue;
   if (use_split(L, n, solver)) {
     if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, wk, s);
     if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, wk, s);
