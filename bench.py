@@ -50,6 +50,32 @@ def make_letkf_gpu(L, k, p, rho, sigma, seed, dtype):
     return A.to(dtype).contiguous()
 
 
+def householder_q(G):
+    """Q of the QR factorisation of each matrix of G (B, n, n), by Householder
+    reflections written with batched torch ops (no solver library, so the
+    generator also runs under hardware-counter profiling)."""
+    import torch
+    B, n, _ = G.shape
+    A = G.clone()
+    vs = []
+    for k in range(n - 1):
+        x = A[:, k:, k]
+        alpha = torch.linalg.vector_norm(x, dim=1)
+        sgn = torch.where(x[:, 0] >= 0, 1.0, -1.0).to(A.dtype)
+        v = x.clone()
+        v[:, 0] += sgn * alpha
+        v = v / torch.linalg.vector_norm(v, dim=1, keepdim=True).clamp_min(1e-300)
+        sub = A[:, k:, k:]
+        sub -= 2.0 * v.unsqueeze(2) * torch.bmm(v.unsqueeze(1), sub)
+        vs.append(v)
+    Q = torch.eye(n, dtype=A.dtype, device=A.device).expand(B, n, n).contiguous()
+    for k in reversed(range(n - 1)):
+        v = vs[k]
+        sub = Q[:, k:, :]
+        sub -= 2.0 * v.unsqueeze(2) * torch.bmm(v.unsqueeze(1), sub)
+    return Q
+
+
 def make_qdq_gpu(L, n, seed, dtype, frac=0.05, gap=1e-8):
     """A = Q diag(lam) Q^T, Q from the QR of a Gaussian matrix, lam ~ U[-1,1]
     with a fraction `frac` of the eigenvalues replaced by pairs split by `gap`."""
@@ -58,11 +84,11 @@ def make_qdq_gpu(L, n, seed, dtype, frac=0.05, gap=1e-8):
     g.manual_seed(seed)
     A = torch.empty((L, n, n), dtype=torch.float64, device="cuda")
     npairs = max(1, int(round(frac * n / 2)))
-    chunk = 64                      # batched QR workspace grows with the batch
+    chunk = 8192                    # few launches: the generator also runs under profiling
     for s in range(0, L, chunk):
         e = min(L, s + chunk)
         G = torch.randn((e - s, n, n), generator=g, device="cuda", dtype=torch.float64)
-        Q, _ = torch.linalg.qr(G)
+        Q = householder_q(G)
         lam = torch.rand((e - s, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
         idx = torch.argsort(torch.rand((e - s, n), generator=g, device="cuda"), dim=1)[:, : 2 * npairs]
         src = torch.gather(lam, 1, idx[:, 0::2])
@@ -104,11 +130,12 @@ def time_solver(a0, solver, steps, warmup):
 
 
 def time_phases(a0, steps, warmup):
-    """Per-phase HIP-event times (ms) of the split n <= 64 path: the phases are
-    issued as separate library calls on one stream with events between them
-    (tri: eig_tri_kernel; ql: eig_ql_lane_kernel; apply: eig_apply_kernel plus
-    the overflow-fallback launch, whose workgroups exit at once when no matrix
-    overflowed).  Returns {phase: mean ms}."""
+    """Per-phase HIP-event times (ms) of the split paths: the phases are issued
+    as separate library calls on one stream with events between them (n <= 64:
+    tri = eig_tri_kernel, ql = eig_ql_reg_kernel, apply = eig_apply_kernel plus
+    the overflow-fallback launch; larger n: eig_tri_large_kernel,
+    eig_ql_lane_kernel, eig_apply_large_kernel plus the fallback launch, whose
+    workgroups exit at once when no matrix overflowed).  Returns {phase: mean ms}."""
     import numpy as np
     import torch
     import eigeng_batched as eg
@@ -161,7 +188,7 @@ def make_uneven_gpu(count, nlo, nhi, seed, lam_hi=1.0, lam_lo=1e-6):
         for s0 in range(0, c, 256):
             e0 = min(c, s0 + 256)
             G = torch.randn((e0 - s0, n, n), generator=g, device="cuda", dtype=torch.float64)
-            Q, _ = torch.linalg.qr(G)
+            Q = householder_q(G)
             perm = torch.argsort(torch.rand((e0 - s0, n), generator=g, device="cuda"), dim=1)
             lam = lam0[perm]
             M = torch.bmm(Q * lam[:, None, :], Q.transpose(1, 2).contiguous())
@@ -247,6 +274,48 @@ def bench_uneven(count, nlo, nhi, steps, warmup, seed=0xE16E0003):
                          "max_rel_residual_F": worst_rel, "max_orth": worst_orth}}
 
 
+def _traffic(name):
+    """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    of known size) from profiles/, or None."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def bench_n256(steps, warmup):
+    """L = 8192, n = 256, FP64 line: whole-call time, per-phase times, roofline,
+    accuracy of every matrix."""
+    import torch
+    L2, n2 = 8192, 256
+    q0 = make_qdq_gpu(L2, n2, seed=0xE16E0004, dtype=torch.float64)
+    ms2, _, a2, w2, info2 = time_solver(q0, 0, steps, warmup)
+    acc2 = accuracy_all(q0, a2, w2)
+    bad = int((info2 != 0).sum().item())
+    del a2, w2
+    torch.cuda.empty_cache()
+    ph2, a2, w2, _ = time_phases(q0, 2, 1)
+    del q0, a2, w2
+    torch.cuda.empty_cache()
+    kms2 = sum(ph2.values())
+    f2 = (10.0 / 3.0) * n2 ** 3
+    b2 = 8 * (n2 * (n2 + 1) / 2 + n2 * n2 + n2)
+    return {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
+            "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
+            "accuracy_all": acc2,
+            "roofline": {"bound": "mfma", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
+                         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                         "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                         "kernel": "eig_tri_large_kernel + eig_ql_lane_kernel + eig_apply_large_kernel "
+                                   "(+ overflow-fallback launch)",
+                         "kernel_ms": kms2, "phase_ms": ph2,
+                         "note": "(10/3)n^3 flops per matrix against the FP64 vector peak"},
+            "roofline_hbm": {"achieved": L2 * b2 / (kms2 * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "bytes_per_matrix": b2,
+                             "traffic": _traffic("traffic_eig_split_n256_fp64.json")}}
+
+
 def cpu_baseline(a_host, min_seconds=10.0, max_reps=20):
     import numpy as np
     import oracle
@@ -261,6 +330,17 @@ def cpu_baseline(a_host, min_seconds=10.0, max_reps=20):
         if el >= min_seconds or reps >= max_reps:
             break
     return total / el, threads, reps, el, int(np.count_nonzero(info))
+
+
+def accuracy_all(a0, z, w):
+    """The reference checker's metrics (eigen_GPU_check.cu:34-81) for every
+    matrix of the batch, on the GPU (eigen_GPU_check_x_*)."""
+    import eigeng_batched as eg
+    out = eg.check(a0, w, z)
+    wmax = w.abs().amax(dim=1).double()
+    return dict(matrices=int(out.shape[0]), checker_failures=int((out[:, 3] != 0).sum().item()),
+                max_err1_rel=float((out[:, 0] / wmax).max().item()),
+                max_orth=float(out[:, 1].max().item()), TOL=float(out[0, 2].item()))
 
 
 def accuracy_sample(a0_host, z_host, w_host, count=256):
@@ -294,6 +374,8 @@ def main():
     ap.add_argument("--no-n256", action="store_true", help="skip the n=256 FP64 secondary line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch secondary line")
+    ap.add_argument("--n256-only", action="store_true",
+                    help="run only the n=256 FP64 line (for profiling its kernels)")
     ap.add_argument("--uneven-count", type=int, default=100000)
     args = ap.parse_args()
 
@@ -307,6 +389,9 @@ def main():
     import torch
     import eigeng_batched as eg
 
+    if args.n256_only:
+        print(json.dumps({"n256": bench_n256(args.steps, args.warmup)}))
+        return
     dt = torch.float32 if args.dtype == "fp32" else torch.float64
     L, n = args.L, args.n
     k_mem, p_obs, rho, sigma = n, 400, 1.1, 0.1
@@ -331,7 +416,7 @@ def main():
         # the step is a sequence of kernels: the roof is taken over their summed
         # event-timed durations (the fallback launch included in "apply")
         kern_ms = sum(phases.values())
-        kernel = "eig_tri_kernel + eig_ql_lane_kernel + eig_apply_kernel (+ overflow-fallback launch)"
+        kernel = "eig_tri_kernel + eig_ql_reg_kernel + eig_apply_kernel (+ overflow-fallback launch)"
     else:
         kern_ms = ms
         kernel = "one solver kernel per step"
@@ -372,18 +457,9 @@ def main():
         "nonconverged": nbad,
         "accuracy": acc,
     }
+    out["accuracy_all"] = accuracy_all(a0, a, w)
     if not args.no_n256:
-        L2, n2 = 8192, 256
-        q0 = make_qdq_gpu(L2, n2, seed=0xE16E0004, dtype=torch.float64)
-        ms2, _, a2, w2, info2 = time_solver(q0, args.solver, 2, 1)
-        acc2 = accuracy_sample(q0.cpu().numpy(), a2.cpu().numpy(), w2.cpu().numpy(), count=32)
-        del q0, a2
-        out["n256"] = {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs "
-                                   "split by 1e-8", "value": L2 / (ms2 * 1e-3), "unit": "matrices/s",
-                       "ms_per_step": ms2, "nonconverged": int((info2 != 0).sum().item()),
-                       "accuracy": acc2,
-                       "roofline_fp64": {"achieved": L2 / (ms2 * 1e-3) * (10.0 / 3.0) * n2 ** 3 / 1e12,
-                                         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}}
+        out["n256"] = bench_n256(3, 1)
     if not args.no_uneven:
         del a, a0
         torch.cuda.empty_cache()
