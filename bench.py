@@ -274,6 +274,25 @@ def bench_uneven(count, nlo, nhi, steps, warmup, seed=0xE16E0003):
                          "max_rel_residual_F": worst_rel, "max_orth": worst_orth}}
 
 
+def bench_letkf48(steps, warmup):
+    """The learned-warm-start workload's shape solved by the plain solver: one
+    time step of L = 65536 LETKF matrices, n = 48, FP32, k = 48 members,
+    p = 300 observations, rho = 1.05, Y ~ N(0, 0.1^2)."""
+    import torch
+    L, n = 65536, 48
+    a0 = make_letkf_gpu(L, n, 300, 1.05, 0.1, seed=0xE16E0005, dtype=torch.float32)
+    ms, _, a, w, info = time_solver(a0, 0, steps, warmup)
+    acc = accuracy_all(a0, a, w)
+    bad = int((info != 0).sum().item())
+    del a0, a, w
+    torch.cuda.empty_cache()
+    return {"workload": "one step of the drifting-ensemble stream: L=65536 LETKF matrices n=48 FP32, "
+                        "k=48 p=300 rho=1.05 sigma=0.1, plain solver (no learned warm start)",
+            "value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms, "nonconverged": bad,
+            "accuracy_all": acc,
+            "achieved_tflops": L * (10.0 / 3.0) * n ** 3 / (ms * 1e-3) / 1e12}
+
+
 def _traffic(name):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
     of known size) from profiles/, or None."""
@@ -374,6 +393,7 @@ def main():
     ap.add_argument("--no-n256", action="store_true", help="skip the n=256 FP64 secondary line")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch secondary line")
+    ap.add_argument("--no-letkf48", action="store_true", help="skip the L=65536 n=48 line")
     ap.add_argument("--n256-only", action="store_true",
                     help="run only the n=256 FP64 line (for profiling its kernels)")
     ap.add_argument("--uneven-count", type=int, default=100000)
@@ -460,6 +480,8 @@ def main():
     out["accuracy_all"] = accuracy_all(a0, a, w)
     if not args.no_n256:
         out["n256"] = bench_n256(3, 1)
+    if not args.no_letkf48:
+        out["letkf48"] = bench_letkf48(5, 1)
     if not args.no_uneven:
         del a, a0
         torch.cuda.empty_cache()

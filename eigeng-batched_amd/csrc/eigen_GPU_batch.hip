@@ -117,9 +117,12 @@ int global_grid(int L) { return L < 1024 ? L : 1024; }
 // small fixed block keeps the reference's "wk == NULL means query" protocol
 // yielding a non-NULL allocation); the global kernel needs one n x n
 // accumulator per resident workgroup.
+int tuned_solver(int L, int n, int tb, int solver);
+
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
   (void)nm; (void)m;
+  solver = tuned_solver(L, n, tbytes, solver);
   if (use_split(L, n, solver)) return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
   if (use_wave_kernel(n, tbytes)) return WS_HEAD;
   if (n > GL_NMAX) return 0;
@@ -311,10 +314,52 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
   return hipGetLastError();
 }
 
+// ---- autotuned launch shapes ------------------------------------------------
+// Tuning bits found by eigen_GPU_batch_tune_* per (device, element size, n,
+// solver, power-of-two bucket of L); used when a call brings no tuning bits.
+
+struct TuneKey {
+  int dev, tb, n, solver, lb;
+  bool operator<(const TuneKey &o) const
+  {
+    if (dev != o.dev) return dev < o.dev;
+    if (tb != o.tb) return tb < o.tb;
+    if (n != o.n) return n < o.n;
+    if (solver != o.solver) return solver < o.solver;
+    return lb < o.lb;
+  }
+};
+std::mutex tune_mu;
+std::map<TuneKey, int> tune_table;
+
+int l_bucket(int L)
+{
+  int b = 0;
+  while ((1 << (b + 1)) <= L && b < 30) ++b;
+  return b;
+}
+
+TuneKey tune_key(int L, int n, int tb, int solver)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return TuneKey{dev, tb, n, solver & 0xff, l_bucket(L)};
+}
+
+int tuned_solver(int L, int n, int tb, int solver)
+{
+  if (solver & ~0xff) return solver;                 // the caller chose
+  std::lock_guard<std::mutex> lk(tune_mu);
+  if (tune_table.empty()) return solver;
+  auto it = tune_table.find(tune_key(L, n, tb, solver));
+  return it == tune_table.end() ? solver : (solver | it->second);
+}
+
 template <typename T>
 hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s, int solver, int *info)
 {
   if (bad_args(L, nm, n, m)) return hipErrorInvalidValue;
+  solver = tuned_solver(L, n, (int)sizeof(T), solver);
   const int base = solver & 0xff;
   if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2 && base != SOLVER_JACOBI) return hipErrorNotSupported;
   if (wk == nullptr) return hipErrorInvalidValue;
@@ -342,6 +387,91 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (n > GL_NMAX) return hipErrorNotSupported;
   if (use_split_large(L, n, (int)sizeof(T), solver)) return launch_split_large<T>(L, nm, n, a, w, info, solver, wk, s);
   return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
+}
+
+// ---- autotuner --------------------------------------------------------------
+
+// Candidate tuning bits for (n, element size, solver): the n <= 64 fused and
+// split paths; for the large orders the workgroup kernel in two widths and the
+// split path in three tridiagonalisation shapes and two grid sizes.
+std::vector<int> tune_candidates(int L, int n, int tb, int solver)
+{
+  const int base = solver & 0xff;
+  std::vector<int> c;
+  if (base == SOLVER_JACOBI) return c;
+  if (use_wave_kernel(n, tb)) {
+    c.push_back(FLAG_FUSED);
+    if (n <= 64) c.push_back(FLAG_SPLIT);
+    return c;
+  }
+  if (n > GL_NMAX) return c;
+  c.push_back(FLAG_FUSED | (1 << GL_NT_SHIFT));
+  c.push_back(FLAG_FUSED | (2 << GL_NT_SHIFT));
+  const int cfgs = n <= 256 ? 3 : 2;
+  for (int cfg = 1; cfg <= cfgs; ++cfg) {
+    const int cf = (n > 256 && cfg == 2) ? 3 : cfg;
+    c.push_back(FLAG_SPLIT | (cf << TRI_CFG_SHIFT));
+    if (L > 512) c.push_back(FLAG_SPLIT | (cf << TRI_CFG_SHIFT) | (4 << GL_GRID_SHIFT));
+  }
+  return c;
+}
+
+size_t tune_bufsize(int L, int nm, int n, int m, int tb, int solver)
+{
+  size_t mx = WS_HEAD;
+  for (int c : tune_candidates(L, n, tb, solver)) {
+    const size_t b = workspace_bytes(L, nm, n, m, tb, (solver & 0xff) | c);
+    mx = b > mx ? b : mx;
+  }
+  return mx;
+}
+
+template <typename T>
+hipError_t run_tune(int L, int nm, int n, const T *a_src, T *a, T *w, void *wk, hipStream_t s, int solver, int *best,
+                    float *best_ms, int max_report, int *cand, float *cand_ms, int *ncand)
+{
+  if (bad_args(L, nm, n, n) || !a_src || !a || !w || !wk || !best) return hipErrorInvalidValue;
+  const std::vector<int> cands = tune_candidates(L, n, (int)sizeof(T), solver);
+  if (cands.empty()) return hipErrorNotSupported;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return hipErrorOutOfMemory;
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return hipErrorOutOfMemory; }
+  const size_t bytes = (size_t)L * nm * n * sizeof(T);
+  hipError_t err = hipSuccess;
+  int bi = -1;
+  float bms = 0.0f;
+  int nrep = 0;
+  for (size_t i = 0; i < cands.size() && err == hipSuccess; ++i) {
+    const int sv = (solver & 0xff) | cands[i];
+    float tmin = -1.0f;
+    for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {   // one warm-up, two timed
+      err = hipMemcpyAsync(a, a_src, bytes, hipMemcpyDeviceToDevice, s);
+      if (err == hipSuccess) err = hipEventRecord(e0, s);
+      if (err == hipSuccess) err = run<T>(L, nm, n, n, a, w, wk, s, sv, nullptr);
+      if (err == hipSuccess) err = hipEventRecord(e1, s);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err == hipSuccess && rep > 0) tmin = (tmin < 0.0f || ms < tmin) ? ms : tmin;
+    }
+    if (err != hipSuccess) break;
+    if (cand && cand_ms && nrep < max_report) { cand[nrep] = cands[i]; cand_ms[nrep] = tmin; ++nrep; }
+    if (bi < 0 || tmin < bms) { bi = (int)i; bms = tmin; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ncand) *ncand = nrep;
+  if (err != hipSuccess) return err;
+  *best = cands[bi];
+  if (best_ms) *best_ms = bms;
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_table[tune_key(L, n, (int)sizeof(T), solver)] = cands[bi];
+  }
+  // leave a holding the eigenvectors of the chosen configuration
+  err = hipMemcpyAsync(a, a_src, bytes, hipMemcpyDeviceToDevice, s);
+  if (err == hipSuccess) err = run<T>(L, nm, n, n, a, w, wk, s, (solver & 0xff) | cands[bi], nullptr);
+  return err;
 }
 
 // ---- uneven batches ---------------------------------------------------------
@@ -570,6 +700,34 @@ int eigen_GPU_check_x_FP(int L, int nm, int n, int m, const float *a, const floa
 {
   if (!out) return (int)hipErrorInvalidValue;
   return (int)run_check<float>(L, nm, n, m, a, w, z, out, 0, stream);
+}
+
+size_t eigen_GPU_batch_tune_bufsize(int L, int nm, int n, int dtype_bytes, int solver)
+{
+  if (bad_args(L, nm, n, n) || (dtype_bytes != 4 && dtype_bytes != 8)) return 0;
+  return tune_bufsize(L, nm, n, n, dtype_bytes, solver);
+}
+
+int eigen_GPU_batch_tune_FP(int L, int nm, int n, const float *a_src, float *a, float *w, void *wk,
+                            hipStream_t stream, int solver, int *best, float *best_ms, int max_report, int *cand,
+                            float *cand_ms, int *ncand)
+{
+  return (int)run_tune<float>(L, nm, n, a_src, a, w, wk, stream, solver, best, best_ms, max_report, cand, cand_ms,
+                              ncand);
+}
+
+int eigen_GPU_batch_tune_DP(int L, int nm, int n, const double *a_src, double *a, double *w, void *wk,
+                            hipStream_t stream, int solver, int *best, float *best_ms, int max_report, int *cand,
+                            float *cand_ms, int *ncand)
+{
+  return (int)run_tune<double>(L, nm, n, a_src, a, w, wk, stream, solver, best, best_ms, max_report, cand, cand_ms,
+                               ncand);
+}
+
+void eigen_GPU_batch_tune_reset(void)
+{
+  std::lock_guard<std::mutex> lk(tune_mu);
+  tune_table.clear();
 }
 
 const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.1"; }

@@ -65,6 +65,14 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp]
             f.restype = None
+        for name in ("eigen_GPU_batch_tune_FP", "eigen_GPU_batch_tune_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, ci, vp, vp, vp, vp, vp, ci, vp, vp, ci, vp, vp, vp]
+            f.restype = ci
+        L.eigen_GPU_batch_tune_bufsize.argtypes = [ci, ci, ci, ci, ci]
+        L.eigen_GPU_batch_tune_bufsize.restype = sz
+        L.eigen_GPU_batch_tune_reset.argtypes = []
+        L.eigen_GPU_batch_tune_reset.restype = None
         L.eigen_GPU_batch_uneven_bufsize.argtypes = [ci, ci, ci, ci]
         L.eigen_GPU_batch_uneven_bufsize.restype = sz
         L.eigen_GPU_batch_bufsize.argtypes = [ci, ci, ci, ci, ci, ci]
@@ -227,6 +235,38 @@ def check(a_full, w, z, m: int | None = None, stream=None):
     if rc != 0:
         raise RuntimeError(f"eigen_GPU_check_x failed: hipError {rc}")
     return out
+
+
+def autotune(a_src, solver: int = IMTQL2, stream=None):
+    """Run the library's autotuner on the batch a_src (L, n, nm), which is left
+    untouched.  Returns (best_bits, best_ms, {bits: ms}, a, w): the chosen
+    tuning bits (remembered by the library for later calls of this shape), the
+    candidate times, and the eigenpairs of the chosen configuration."""
+    import torch
+    if not a_src.is_cuda:
+        raise ValueError("autotune needs a device tensor")
+    L, n, nm = a_src.shape
+    tb = a_src.element_size()
+    a = torch.empty_like(a_src)
+    w = torch.empty((L, n), dtype=a_src.dtype, device=a_src.device)
+    ws = torch.empty(max(int(lib().eigen_GPU_batch_tune_bufsize(L, nm, n, tb, solver)), 1),
+                     dtype=torch.uint8, device=a_src.device)
+    best = ctypes.c_int(0)
+    best_ms = ctypes.c_float(0.0)
+    cand = (ctypes.c_int * 32)()
+    cms = (ctypes.c_float * 32)()
+    nc = ctypes.c_int(0)
+    f = lib().eigen_GPU_batch_tune_FP if tb == 4 else lib().eigen_GPU_batch_tune_DP
+    rc = f(L, nm, n, a_src.data_ptr(), a.data_ptr(), w.data_ptr(), ws.data_ptr(), _stream_ptr(stream),
+           solver, ctypes.addressof(best), ctypes.addressof(best_ms), 32, ctypes.addressof(cand),
+           ctypes.addressof(cms), ctypes.addressof(nc))
+    if rc != 0:
+        raise RuntimeError(f"eigen_GPU_batch_tune failed: hipError {rc}")
+    return best.value, best_ms.value, {cand[i]: cms[i] for i in range(nc.value)}, a, w
+
+
+def tune_reset() -> None:
+    lib().eigen_GPU_batch_tune_reset()
 
 
 def version() -> str:

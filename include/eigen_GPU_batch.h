@@ -142,6 +142,29 @@ int eigen_GPU_check_x_DP(int L, int nm, int n, int m, const double *a, const dou
 int eigen_GPU_check_x_FP(int L, int nm, int n, int m, const float *a, const float *w,
                          const float *z, double *out, hipStream_t stream);
 
+/*
+ * Autotuner (no reference symbol; the reference picks launch shapes by fixed
+ * heuristics, eigen_GPU_batch.hpp:23-102).  Times every candidate tuning-bit
+ * set for (L, nm, n, element type, solver) on the caller's batch: a_src holds
+ * the L input matrices (left untouched), a is a work copy that is restored from
+ * a_src before each trial and ends holding the eigenvectors of the chosen set,
+ * w the eigenvalues.  The winner is returned in *best (its time in *best_ms)
+ * and remembered for this device: later calls with the same n, element type and
+ * solver, an L in the same power-of-two range and no tuning bits of their own
+ * use it (workspace queries included).  cand / cand_ms (may be NULL) receive up
+ * to max_report candidates and their times, *ncand their number.  wk must hold
+ * eigen_GPU_batch_tune_bufsize(...) bytes.  Synchronizes stream.
+ */
+size_t eigen_GPU_batch_tune_bufsize(int L, int nm, int n, int dtype_bytes, int solver);
+int eigen_GPU_batch_tune_FP(int L, int nm, int n, const float *a_src, float *a, float *w,
+                            void *wk, hipStream_t stream, int solver, int *best, float *best_ms,
+                            int max_report, int *cand, float *cand_ms, int *ncand);
+int eigen_GPU_batch_tune_DP(int L, int nm, int n, const double *a_src, double *a, double *w,
+                            void *wk, hipStream_t stream, int solver, int *best, float *best_ms,
+                            int max_report, int *cand, float *cand_ms, int *ncand);
+/* Forget every tuned choice (all devices). */
+void eigen_GPU_batch_tune_reset(void);
+
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);
 

@@ -285,3 +285,27 @@ def test_invalid_arguments_are_noops():
         assert rc != 0
     torch.cuda.synchronize()
     assert torch.all(a == 1) and torch.all(w == 0)
+
+
+@pytest.mark.parametrize("n,dtype,L", [(48, np.float32, 2500), (200, np.float64, 600)])
+def test_autotune_picks_a_candidate_and_stays_correct(n, dtype, L):
+    """The autotuner times its candidates on the batch, returns one of them,
+    leaves a_src untouched and the chosen configuration's eigenpairs correct;
+    a later default call of the same shape still solves correctly."""
+    print(f"[tune] n={n} dtype={np.dtype(dtype).name} L={L}", flush=True)
+    eg.tune_reset()
+    a0 = wl.sym_rand(L, n, n, dtype=dtype, seed=900 + n)
+    src = torch.from_numpy(a0.copy()).to(DEV)
+    best, best_ms, table, z, w = eg.autotune(src)
+    torch.cuda.synchronize()
+    assert best in table and best_ms == pytest.approx(min(table.values()))
+    assert len(table) >= 2
+    assert torch.equal(src.cpu(), torch.from_numpy(a0))
+    full = torch.from_numpy(np.ascontiguousarray(np.transpose(wl.unpack(a0), (0, 2, 1)))).to(DEV)
+    out = eg.check(full, w, z).cpu().numpy()
+    assert (out[:, 3] == 0).all()
+    z2, w2, info = run_gpu(a0)
+    assert info.max() == 0
+    out2 = eg.check(full, torch.from_numpy(w2).to(DEV), torch.from_numpy(z2).to(DEV)).cpu().numpy()
+    assert (out2[:, 3] == 0).all()
+    eg.tune_reset()
