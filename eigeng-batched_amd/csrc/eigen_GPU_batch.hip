@@ -104,11 +104,13 @@ bool use_wave_kernel(int n, int tbytes) { return tbytes == 4 ? n <= 128 : n <= 6
 bool use_split_large(int L, int n, int tbytes, int solver)
 {
   const int base = solver & 0xff, flags = solver & ~0xff;
-  if (use_wave_kernel(n, tbytes) || n > GL_NMAX) return false;
+  if (n <= 64 || n > GL_NMAX) return false;
   if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2) return false;
   if (flags & FLAG_FUSED) return false;
   if (flags & FLAG_SPLIT) return true;
-  return L >= SPLIT_LARGE_MIN_L;
+  // FP32 64 < n <= 128 also has the two-wave register kernel, faster for small
+  // batches; measured at L = 8192: n = 96 25 ms split / 36 ms fused, n = 128 40 / 59 ms
+  return L >= (use_wave_kernel(n, tbytes) ? SPLIT_MIN_L : SPLIT_LARGE_MIN_L);
 }
 
 int global_grid(int L) { return L < 1024 ? L : 1024; }
@@ -124,10 +126,10 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
   (void)nm; (void)m;
   solver = tuned_solver(L, n, tbytes, solver);
   if (use_split(L, n, solver)) return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
-  if (use_wave_kernel(n, tbytes)) return WS_HEAD;
   if (n > GL_NMAX) return 0;
   if (use_split_large(L, n, tbytes, solver))
     return WS_HEAD + SplitLargeLayout(L < LCHUNK_LARGE ? L : LCHUNK_LARGE, n, tbytes).total;
+  if (use_wave_kernel(n, tbytes)) return WS_HEAD;
   return WS_HEAD + (size_t)global_grid(L) * (size_t)n * (size_t)n * (size_t)tbytes;
 }
 
@@ -382,10 +384,10 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, solver, s);
   if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
   if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
-  if (use_wave_kernel(n, (int)sizeof(T)))
-    return launch_wave<T, (sizeof(T) == 4 ? 128 : 64)>(L, nm, n, a, w, info, solver, s);
   if (n > GL_NMAX) return hipErrorNotSupported;
   if (use_split_large(L, n, (int)sizeof(T), solver)) return launch_split_large<T>(L, nm, n, a, w, info, solver, wk, s);
+  if (use_wave_kernel(n, (int)sizeof(T)))
+    return launch_wave<T, (sizeof(T) == 4 ? 128 : 64)>(L, nm, n, a, w, info, solver, s);
   return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
 }
 
@@ -401,7 +403,11 @@ std::vector<int> tune_candidates(int L, int n, int tb, int solver)
   if (base == SOLVER_JACOBI) return c;
   if (use_wave_kernel(n, tb)) {
     c.push_back(FLAG_FUSED);
-    if (n <= 64) c.push_back(FLAG_SPLIT);
+    if (n <= 64) {
+      c.push_back(FLAG_SPLIT);
+      return c;
+    }
+    for (int cfg = 1; cfg <= 3; ++cfg) c.push_back(FLAG_SPLIT | (cfg << TRI_CFG_SHIFT));
     return c;
   }
   if (n > GL_NMAX) return c;

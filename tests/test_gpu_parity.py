@@ -160,8 +160,6 @@ def test_large_split_vs_oracle(n, dtype, solver, path):
     """Orders beyond the register kernels through the split path (workgroup
     tridiagonalisation, lane-per-matrix QL with rotation log, row-block apply)
     and through its overflow fallback (log capacity forced to 1)."""
-    if dtype == np.float32 and n <= 128:
-        pytest.skip("FP32 n <= 128 is served by the register kernels")
     print(f"[parity] large {path} solver={solver} n={n} dtype={np.dtype(dtype).name}", flush=True)
     nm = n + (n & 1)
     L = 7 if n <= 256 else 3
