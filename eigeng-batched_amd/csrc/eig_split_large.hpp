@@ -400,41 +400,61 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
           }
         }
       } else {
-        // FP64: only the pairs inside the window (the branch-free form needs
-        // more registers than the doubles of the chunk leave free)
-        T h1 = T(0);
-        bool live = false;
-        if (enters) {
-          h1 = S.carry[q][lane];
-          live = true;
-          if (ilo == phi + 1) {                      // the chain ended on the pair above
-            x[CW - 1] = h1;
-            live = false;
-          }
-        } else if (m - 1 >= plo && m - 1 <= phi) {
-#pragma unroll
-          for (int j = 0; j < CW; ++j)
-            if (c0 + j == m) h1 = x[j];
-          live = true;
-        }
-        if (live) {
-          const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
-#pragma unroll
-          for (int p = CW - 1; p >= 0; --p) {
-            const int i = c0 - 1 + p;
-            if (i <= top && i >= bot) {
-              const T2 g = S.rs[q][p];
-              const T h0 = (p >= 1) ? x[p > 0 ? p - 1 : 0] : xlow;
-              x[p] = g.y * h0 + g.x * h1;
-              h1 = g.x * h0 - g.y * h1;
-            }
-          }
-          if (bot == c0 - 1 && q > 0) {
-            S.carry[q - 1][lane] = h1;               // pending (or final) column c0-1
-          } else if (bot == ilo) {
+        // FP64: 4-pair blocks from the one holding the window's top pair down to
+        // the one holding its bottom pair; the rotations of a block are read
+        // together (the next block's while the current one is applied), and the
+        // identity rotations outside the window (see the FP32 form above) make
+        // every pair of a block safe to apply.
+        if (enters || top >= bot) {
+          constexpr int PB = 4;
+          const int ptop = enters ? CW - 1 : top - (c0 - 1);   // highest pair index p to apply
+          const int pbot = bot - (c0 - 1);                      // lowest pair index p in the window
+          const int btop = ptop / PB, bbot = pbot / PB;
+          T h1 = T(0);
+          if (enters) {
+            h1 = S.carry[q][lane];
+          } else {
 #pragma unroll
             for (int j = 0; j < CW; ++j)
-              if (c0 + j == ilo) x[j] = h1;
+              if (j == btop * PB + PB - 1) h1 = x[j];
+          }
+          const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
+          T2 g[PB], gn[PB];
+#pragma unroll
+          for (int jj = 0; jj < PB; ++jj) gn[jj] = S.rs[q][btop * PB + jj];
+#pragma unroll
+          for (int b = CW / PB - 1; b >= 0; --b) {
+            if (b <= btop && b >= bbot) {
+#pragma unroll
+              for (int jj = 0; jj < PB; ++jj) g[jj] = gn[jj];
+              if (b > bbot) {
+#pragma unroll
+                for (int jj = 0; jj < PB; ++jj) gn[jj] = S.rs[q][(b - 1) * PB + jj];
+              }
+#pragma unroll
+              for (int jj = PB - 1; jj >= 0; --jj) {
+                const int p = b * PB + jj;
+                if (p >= 1) {
+                  const T h0 = x[p >= 1 ? p - 1 : 0];
+                  x[p] = g[jj].y * h0 + g[jj].x * h1;
+                  h1 = g[jj].x * h0 - g[jj].y * h1;
+                } else if (q > 0) {
+                  x[0] = g[jj].y * xlow + g[jj].x * h1;
+                  h1 = g[jj].x * xlow - g[jj].y * h1;
+                }
+              }
+            }
+          }
+          // where the chain leaves this wave: column c0-1 (wave q-1) takes the
+          // pending value; otherwise, if the window's bottom pair was the lowest
+          // pair applied, its pending value is the final column ilo
+          const int plow = (bbot * PB > (q > 0 ? 0 : 1)) ? bbot * PB : (q > 0 ? 0 : 1);
+          if (q > 0 && pbot == 0) {
+            S.carry[q - 1][lane] = h1;
+          } else if (plow == pbot && bot == ilo) {
+#pragma unroll
+            for (int j = 0; j < CW; ++j)
+              if (j == pbot - 1) x[j] = h1;
           }
         }
       }
