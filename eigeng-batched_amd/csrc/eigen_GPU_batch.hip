@@ -289,7 +289,7 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
     }
     if (!(ph & FLAG_PH_QL)) {
     } else if (n <= 256) {
-      constexpr int NLANE = 16;
+      constexpr int NLANE = sizeof(T) == 8 ? 32 : 64;
       hipLaunchKernelGGL((eig_ql_lane_kernel<T, 256, NLANE>), dim3((cur + NLANE - 1) / NLANE), dim3(64), 0, s,
                          cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, sv);
     } else {
