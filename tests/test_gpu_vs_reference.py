@@ -88,3 +88,36 @@ def test_matches_reference_on_frank(reflib, dtype):
     eps = 2.0 ** -52 if dtype == np.float64 else 2.0 ** -23
     TOL = eps * (1023.0 if dtype == np.float64 else 31.0) * np.sqrt(n + 16.0)
     assert np.max(np.abs(wm - wr) / wr[:, -1:]) <= TOL
+
+
+def run_mine_path(a_np, path):
+    t = torch.from_numpy(np.ascontiguousarray(a_np)).cuda()
+    t, w, info = eg.eigen_batch(t, path=path)
+    torch.cuda.synchronize()
+    assert int(info.max()) == 0
+    return t.cpu().numpy(), w.cpu().numpy()
+
+
+@pytest.mark.parametrize("dtype,n,path", [(np.float32, 64, "split"), (np.float32, 48, "split"),
+                                          (np.float64, 32, "split"), (np.float64, 64, "split"),
+                                          (np.float64, 200, "split"), (np.float32, 100, "split"),
+                                          (np.float32, 256, "split")])
+def test_split_paths_match_reference(reflib, n, dtype, path):
+    """The split paths (register QL, rotation logs, large-order pipeline) against
+    the reference library on the same LETKF-like inputs."""
+    print(f"[vs-ref] split n={n} dtype={np.dtype(dtype).name}", flush=True)
+    a0, _ = wl.letkf_y(40 if n <= 128 else 8, k=n, p=n + 100, rho=1.05, sigma=0.1, dtype=dtype, seed=77 + n)
+    zr, wr = run_ref(reflib, a0)
+    zm, wm = run_mine_path(a0, path)
+    eps = 2.0 ** -52 if dtype == np.float64 else 2.0 ** -23
+    TOL = eps * (1023.0 if dtype == np.float64 else 31.0) * np.sqrt(n + 16.0)
+    scale = np.maximum(np.abs(wr[:, :1]), np.abs(wr[:, -1:]))
+    assert np.max(np.abs(wm - wr) / scale) <= TOL
+    gap = np.full(wr.shape, np.inf)
+    dw = np.diff(wr, axis=1)
+    gap[:, :-1] = np.minimum(gap[:, :-1], dw)
+    gap[:, 1:] = np.minimum(gap[:, 1:], dw)
+    sep = gap / scale > 1e3 * np.sqrt(eps)
+    dots = np.abs(np.einsum("kjr,kjr->kj", zm[:, :, :n].astype(np.float64), zr[:, :, :n].astype(np.float64)))
+    bound = 8.0 * n * eps / np.maximum(gap / scale, 1e-300)
+    assert not (sep & (1.0 - dots > bound)).any()
