@@ -246,20 +246,271 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
   __syncthreads();
 }
 
+// One step of a butterfly reduce-scatter: a lane and its partner under Mv
+// hold 2K partial values for the same 2K indices and opposite `side`s; the lane
+// keeps half `side` (indices [K side, K side + K)) with the partner's partials
+// added, in v[0..K).
+template <int K, typename T, typename Mv>
+__device__ __forceinline__ void reduce_scatter_step(T *v, bool side, Mv mv)
+{
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const T keep = side ? v[i + K] : v[i];
+    const T send = side ? v[i] : v[i + K];
+    v[i] = keep + lane_move(send, mv);
+  }
+}
+
+// Symmetric-storage form of blocked_tridiag (the default for n <= 256): the
+// live matrix is the lower triangle with the diagonal, loaded as the transpose
+// of the input's upper triangle.  The SYMV reads each lower element once per
+// column (32 x 32 tiles, row and column sums) and the panel update touches only
+// the lower triangle, so both passes move half the bytes of the full form.
+// The strictly upper part of A receives the u_c storage and nothing else; the
+// Householder arithmetic and outputs are those of blocked_tridiag.
 template <typename T, int NT, int NBP, int NMAXT, int RG>
+__device__ void blocked_tridiag_sym(const T *__restrict__ Ain, int nma, T *__restrict__ A, int nm, int n,
+                                    TriBlkLds<T, NBP, NMAXT, RG> &S, T (&pw)[NT / 64][NMAXT])
+{
+  constexpr int RT = NT / RG;
+  static_assert(2 * RT >= NMAXT, "two rows per thread cover the matrix");
+  static_assert(NT % 32 == 0 && NBP * NMAXT >= 32 * 33, "transpose staging fits in U");
+  const int t = threadIdx.x;
+  const int g = t / RT;
+  const int r0 = t - g * RT, r1 = r0 + RT;
+  const bool lead = (g == 0);                        // the group that owns per-row state
+  // lower triangle of A <- transpose of the upper triangle of Ain, in 32 x 32
+  // tiles staged through LDS (coalesced on both sides)
+  {
+    T (*stg)[33] = reinterpret_cast<T (*)[33]>(&S.U[0][0]);
+    const int nt = (n + 31) >> 5, tr = t & 31;
+    for (int I = 0; I < nt; ++I) {
+      for (int J = 0; J <= I; ++J) {
+        for (int tc = t >> 5; tc < 32; tc += NT / 32) {
+          const int ar = 32 * J + tr, ac = 32 * I + tc;   // Ain(ar, ac), upper block (J, I)
+          if (ar < n && ac < n && (I > J || tr <= tc)) stg[tr][tc] = Ain[ar + (size_t)ac * nma];
+        }
+        __syncthreads();
+        for (int tc = t >> 5; tc < 32; tc += NT / 32) {
+          const int wr = 32 * I + tr, wc = 32 * J + tc;   // A(wr, wc) = Ain(wc, wr)
+          if (wr < n && wc < n && (I > J || tr >= tc)) A[wr + (size_t)wc * nm] = stg[tc][tr];
+        }
+        __syncthreads();
+      }
+    }
+  }
+  for (int i = t; i < NMAXT; i += NT) {
+#pragma unroll
+    for (int k = 0; k < NBP; ++k) { S.U[k][i] = T(0); S.V[k][i] = T(0); }
+  }
+  __syncthreads();
+
+  for (int ctop = n - 1; ctop >= 1; ctop -= NBP) {
+    const int nbk = (NBP < ctop) ? NBP : ctop;
+    for (int k = 0; k < nbk; ++k) {
+      const int c = ctop - k;
+      // 1. column c of the updated matrix (rows < c) and its diagonal
+      T uc0 = T(0), uc1 = T(0);
+      if (lead) {
+        T ukc[NBP], vkc[NBP];
+#pragma unroll
+        for (int kk = 0; kk < NBP; ++kk) { ukc[kk] = S.U[kk][c]; vkc[kk] = S.V[kk][c]; }
+        if (r0 <= c) {
+          T acc = A[c + (size_t)r0 * nm];              // row c of the lower triangle
+#pragma unroll
+          for (int kk = 0; kk < NBP; ++kk)
+            if (kk < k) acc = fma_(S.U[kk][r0], vkc[kk], fma_(S.V[kk][r0], ukc[kk], acc));
+          if (r0 < c) uc0 = acc; else S.d[c] = acc;
+        }
+        if (r1 <= c) {
+          T acc = A[c + (size_t)r1 * nm];
+#pragma unroll
+          for (int kk = 0; kk < NBP; ++kk)
+            if (kk < k) acc = fma_(S.U[kk][r1], vkc[kk], fma_(S.V[kk][r1], ukc[kk], acc));
+          if (r1 < c) uc1 = acc; else S.d[c] = acc;
+        }
+        if (r0 < c) S.u[r0] = uc0;
+        if (r1 < c) S.u[r1] = uc1;
+      }
+      T sg[1] = {uc0 * uc0 + uc1 * uc1};
+      block_sums<NT, 1>(sg, 1, S);
+      const T ulast = S.u[c - 1];
+      const T anorm = -sign_(sqrt_(sg[0]), ulast);
+      const T ul = ulast - anorm;
+      const T beta = T(1) / nz(ul * anorm);
+      if (r0 == c - 1) uc0 = ul;
+      if (r1 == c - 1) uc1 = ul;
+      // u_c is stored in A(0:c-1, c) (with u(c-1) = ul), as in the unblocked form
+      if (lead) {
+        if (r0 < c) A[r0 + (size_t)c * nm] = uc0;
+        if (r1 < c) A[r1 + (size_t)c * nm] = uc1;
+      }
+      __syncthreads();
+      if (t == 0) {
+        S.u[c - 1] = ul;
+        S.e[c - 1] = anorm;
+        S.beta[c] = beta;
+      }
+      __syncthreads();
+      // 2. p = beta (A11 u + panel corrections)
+      // A(0:c-1, 0:c-1) u from the lower triangle only, in 32x32 tiles (I, J),
+      // J <= I: a tile adds T u_J to p_I (row sums) and, off the diagonal,
+      // T^T u_I to p_J (column sums over the tile's rows).  Each wave keeps its
+      // own partial vector pw[wave], summed in wave order afterwards, so the
+      // result does not depend on scheduling.
+      {
+        const int lane = t & 63, wv = t >> 6, rr = lane & 31, hh = lane >> 5;
+        for (int i = lane; i < c; i += 64) pw[wv][i] = T(0);
+        wsync();
+        const int nt = (c + 31) >> 5;
+        const int ntiles = nt * (nt + 1) / 2;
+        for (int tile = wv; tile < ntiles; tile += NT / 64) {
+          int I = 0;
+          while ((I + 1) * (I + 2) / 2 <= tile) ++I;
+          const int J = tile - I * (I + 1) / 2;
+          const int row = 32 * I + rr;
+          const int col0 = 32 * J + 16 * hh;
+          const T ur = (row < c) ? S.u[row] : T(0);
+          // every load is issued (clamped into the live block, value dropped
+          // when not live) so the 16 loads go out back to back
+          const T *__restrict__ arow = A + (row < c ? row : c - 1);
+          T srow = T(0);
+          T tc[16];
+          T av[16];
+#pragma unroll
+          for (int cc = 0; cc < 16; ++cc) {
+            const int col = col0 + cc;
+            av[cc] = arow[(size_t)(col < c ? col : c - 1) * nm];
+          }
+#pragma unroll
+          for (int cc = 0; cc < 16; ++cc) {
+            const int col = col0 + cc;
+            const bool live = row < c && col < c && (I > J || col <= row);
+            const T v = live ? av[cc] : T(0);
+            const T uc = (col < c) ? S.u[col] : T(0);
+            srow = fma_(v, uc, srow);
+            tc[cc] = (I > J || col < row) ? v * ur : T(0);
+          }
+          srow += lane_move(srow, MvX32());
+          if (hh == 0 && row < c) pw[wv][row] += srow;
+          wsync();
+          // column sums over the 32 rows: reduce-scatter, lane rr ends with
+          // column rr >> 1 of its half
+          reduce_scatter_step<8>(tc, (rr & 16) != 0, MvX16());
+          reduce_scatter_step<4>(tc, (rr & 8) != 0, MvRM());
+          reduce_scatter_step<2>(tc, (rr & 4) != 0, MvHM());
+          reduce_scatter_step<1>(tc, (rr & 2) != 0, MvX2());
+          tc[0] += lane_move(tc[0], MvX1());
+          if (!(rr & 1) && col0 + (rr >> 1) < c) pw[wv][col0 + (rr >> 1)] += tc[0];
+          wsync();
+        }
+      }
+      __syncthreads();
+      // dots of the panel vectors with u
+      T dd[2 * NBP];
+#pragma unroll
+      for (int kk = 0; kk < 2 * NBP; ++kk) dd[kk] = T(0);
+      if (lead) {
+#pragma unroll
+        for (int kk = 0; kk < NBP; ++kk) {
+          if (kk < k) {
+            if (r0 < c) { dd[2 * kk] = S.V[kk][r0] * uc0; dd[2 * kk + 1] = S.U[kk][r0] * uc0; }
+            if (r1 < c) {
+              dd[2 * kk] = fma_(S.V[kk][r1], uc1, dd[2 * kk]);
+              dd[2 * kk + 1] = fma_(S.U[kk][r1], uc1, dd[2 * kk + 1]);
+            }
+          }
+        }
+      }
+      block_sums<NT, 2 * NBP>(dd, 2 * k, S);
+      T p0 = T(0), p1 = T(0);
+      if (lead) {
+#pragma unroll
+        for (int q = 0; q < NT / 64; ++q) {
+          if (r0 < c) p0 += pw[q][r0];
+          if (r1 < c) p1 += pw[q][r1];
+        }
+      }
+      if (lead) {
+#pragma unroll
+        for (int kk = 0; kk < NBP; ++kk) {
+          if (kk < k) {
+            if (r0 < c) p0 = fma_(S.U[kk][r0], dd[2 * kk], fma_(S.V[kk][r0], dd[2 * kk + 1], p0));
+            if (r1 < c) p1 = fma_(S.U[kk][r1], dd[2 * kk], fma_(S.V[kk][r1], dd[2 * kk + 1], p1));
+          }
+        }
+      }
+      p0 = (lead && r0 < c) ? p0 * beta : T(0);
+      p1 = (lead && r1 < c) ? p1 * beta : T(0);
+      T al[1] = {p0 * uc0 + p1 * uc1};
+      block_sums<NT, 1>(al, 1, S);
+      const T alpha = al[0] * (beta * T(0.5));
+      if (lead) {
+        if (r0 < NMAXT) { S.U[k][r0] = (r0 < c) ? uc0 : T(0); S.V[k][r0] = (r0 < c) ? fma_(alpha, uc0, p0) : T(0); }
+        if (r1 < NMAXT) { S.U[k][r1] = (r1 < c) ? uc1 : T(0); S.V[k][r1] = (r1 < c) ? fma_(alpha, uc1, p1) : T(0); }
+      }
+      __syncthreads();
+    }
+    // 3. panel update of the trailing block A(0:clow-1, 0:clow-1)
+    const int clow = ctop - nbk + 1;
+    {
+      T ur0[NBP], vr0[NBP], ur1[NBP], vr1[NBP];
+#pragma unroll
+      for (int kk = 0; kk < NBP; ++kk) {
+        ur0[kk] = (r0 < clow && kk < nbk) ? S.U[kk][r0] : T(0);
+        vr0[kk] = (r0 < clow && kk < nbk) ? S.V[kk][r0] : T(0);
+        ur1[kk] = (r1 < clow && kk < nbk) ? S.U[kk][r1] : T(0);
+        vr1[kk] = (r1 < clow && kk < nbk) ? S.V[kk][r1] : T(0);
+      }
+      if (r0 < clow) {
+        const int jend = ((r1 < clow) ? r1 : r0) + 1;  // lower triangle and diagonal only
+        for (int j = g; j < jend; j += RG) {
+          const bool w0 = j <= r0;
+          T a0 = w0 ? A[r0 + (size_t)j * nm] : T(0);
+          T a1 = (r1 < clow) ? A[r1 + (size_t)j * nm] : T(0);
+#pragma unroll
+          for (int kk = 0; kk < NBP; ++kk) {
+            if (kk < nbk) {
+              const T uj = S.U[kk][j], vj = S.V[kk][j];
+              a0 = fma_(ur0[kk], vj, fma_(vr0[kk], uj, a0));
+              a1 = fma_(ur1[kk], vj, fma_(vr1[kk], uj, a1));
+            }
+          }
+          if (w0) A[r0 + (size_t)j * nm] = a0;
+          if (r1 < clow) A[r1 + (size_t)j * nm] = a1;
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < NMAXT; i += NT) {
+#pragma unroll
+      for (int k = 0; k < NBP; ++k) { S.U[k][i] = T(0); S.V[k][i] = T(0); }
+    }
+    __syncthreads();
+  }
+  if (t == 0) { S.d[0] = A[0]; S.e[n - 1] = T(0); }
+  __syncthreads();
+}
+
+template <typename T, int NT, int NBP, int NMAXT, int RG, bool SYM = false>
 __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n, const T *__restrict__ a,
                                                            T *__restrict__ W, T *__restrict__ de, int ld,
                                                            T *__restrict__ bws)
 {
   __shared__ TriBlkLds<T, NBP, NMAXT, RG> S;
+  __shared__ T pw[SYM ? NT / 64 : 1][SYM ? NMAXT : 1];
   const int t = threadIdx.x;
   for (long long mat = blockIdx.x; mat < L; mat += gridDim.x) {
     const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
     T *__restrict__ Wm = W + (size_t)mat * (size_t)n * (size_t)n;
-    for (int c = 0; c < n; ++c)
-      for (int r = t; r <= c; r += NT) Wm[r + (size_t)c * n] = A[r + (size_t)c * nm];
-    __syncthreads();
-    blocked_tridiag<T, NT, NBP, NMAXT, RG>(Wm, n, n, S);
+    if constexpr (SYM) {
+      blocked_tridiag_sym<T, NT, NBP, NMAXT, RG>(A, nm, Wm, n, n, S, pw);
+    } else {
+      for (int c = 0; c < n; ++c)
+        for (int r = t; r <= c; r += NT) Wm[r + (size_t)c * n] = A[r + (size_t)c * nm];
+      __syncthreads();
+      blocked_tridiag<T, NT, NBP, NMAXT, RG>(Wm, n, n, S);
+    }
     for (int i = t; i < n; i += NT) {
       de[(size_t)i * ld + mat] = S.d[i];
       de[(size_t)(n + i) * ld + mat] = S.e[i];

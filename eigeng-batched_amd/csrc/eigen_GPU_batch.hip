@@ -207,6 +207,9 @@ constexpr int GL_NT_SHIFT = 16, GL_GRID_SHIFT = 20;
 // 2: 512 threads in 2 column groups, 3: 1024 threads in 4 column groups
 // (2 for n > 256); 0 picks the default
 constexpr int TRI_CFG_SHIFT = 24;
+// bit 26: for n <= 256, the full-storage tridiagonalisation instead of the
+// default symmetric (lower-triangle) one
+constexpr int TRI_FULL_BIT = 1 << 26;
 
 template <typename T>
 hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
@@ -265,10 +268,22 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
     if (ph & FLAG_PH_TRI) {
       // tridiagonalisation shape: workgroup size / column groups and grid
       const int tc = (solver >> TRI_CFG_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
-      const int cfg = tc ? tc : (n <= 256 ? 2 : 1);   // measured best at n = 256 FP64: 512 threads
+      const bool sym = n <= 256 && !(solver & TRI_FULL_BIT);
+      // measured best at n = 128 and 256: symmetric form with 256 threads; full form with 512
+      const int cfg = tc ? tc : (sym ? 1 : (n <= 256 ? 2 : 1));
       unsigned grid = tg;
       if (gc) grid = (unsigned)(128 * gc) < grid ? (unsigned)(128 * gc) : grid;
-      if (n <= 256) {
+      if (sym) {
+        if (cfg == 3)
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, 8, 256, 4, true>), dim3(grid), dim3(1024), 0, s, cur,
+                             nm, n, ac, Wm, de, lc, bw);
+        else if (cfg == 2)
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 512, 8, 256, 2, true>), dim3(grid), dim3(512), 0, s, cur, nm,
+                             n, ac, Wm, de, lc, bw);
+        else
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 256, 1, true>), dim3(grid), dim3(256), 0, s, cur, nm,
+                             n, ac, Wm, de, lc, bw);
+      } else if (n <= 256) {
         if (cfg == 3)
           hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, 8, 256, 4>), dim3(grid), dim3(1024), 0, s, cur, nm, n,
                              ac, Wm, de, lc, bw);
@@ -407,18 +422,22 @@ std::vector<int> tune_candidates(int L, int n, int tb, int solver)
       c.push_back(FLAG_SPLIT);
       return c;
     }
-    for (int cfg = 1; cfg <= 3; ++cfg) c.push_back(FLAG_SPLIT | (cfg << TRI_CFG_SHIFT));
+    c.push_back(FLAG_SPLIT | (1 << TRI_CFG_SHIFT));
+    c.push_back(FLAG_SPLIT | (2 << TRI_CFG_SHIFT));
+    c.push_back(FLAG_SPLIT | TRI_FULL_BIT | (2 << TRI_CFG_SHIFT));
     return c;
   }
   if (n > GL_NMAX) return c;
   c.push_back(FLAG_FUSED | (1 << GL_NT_SHIFT));
   c.push_back(FLAG_FUSED | (2 << GL_NT_SHIFT));
-  const int cfgs = n <= 256 ? 3 : 2;
-  for (int cfg = 1; cfg <= cfgs; ++cfg) {
-    const int cf = (n > 256 && cfg == 2) ? 3 : cfg;
+  // tridiagonalisation shapes: symmetric form 256 / 512 threads and the full
+  // form with 512 threads (n <= 256); 256 / 1024 threads beyond
+  const int cfs[2] = {1, n <= 256 ? 2 : 3};
+  for (int cf : cfs) {
     c.push_back(FLAG_SPLIT | (cf << TRI_CFG_SHIFT));
     if (L > 512) c.push_back(FLAG_SPLIT | (cf << TRI_CFG_SHIFT) | (4 << GL_GRID_SHIFT));
   }
+  if (n <= 256) c.push_back(FLAG_SPLIT | TRI_FULL_BIT | (2 << TRI_CFG_SHIFT));
   return c;
 }
 

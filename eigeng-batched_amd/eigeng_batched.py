@@ -23,8 +23,9 @@ IMTQL2, TQL2, JACOBI = 0, 1, 2
 # tuning flags OR-ed into the solver id of the _x entry points
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
 FLAG_QL_LDS = 0x800
+FLAG_TRI_FULL = 0x4000000
 PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG,
-         "split-lds": FLAG_SPLIT | FLAG_QL_LDS}
+         "split-lds": FLAG_SPLIT | FLAG_QL_LDS, "split-full": FLAG_SPLIT | FLAG_TRI_FULL}
 # split-path phases (a call with a subset runs only those kernels; the phases
 # hand over through the workspace, so tri -> ql -> apply calls on one stream
 # and one workspace compose the full solve, for batches of at most 16384)
@@ -112,7 +113,8 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     "split" (lane-per-matrix QL recurrence) or "split-fallback" (split with the
     rotation log shrunk so that every matrix takes the overflow fallback) or
     "split-lds" (split with the QL recurrence's d, e kept in LDS rather than in
-    registers).
+    registers) or, for 64 < n <= 256, "split-full" (split with the
+    tridiagonalisation on the full matrix instead of its lower triangle).
     phases (split path only): iterable of "tri", "ql", "apply" to run.
     """
     import torch

@@ -65,7 +65,11 @@ enum eigen_GPU_batch_solver {
  *          split path's overflow fallback (a host-forced test of that path);
  *   0x1000 / 0x2000 / 0x4000  run only the split path's tridiagonalisation /
  *          QL / apply phase (L <= 16384): successive calls on one stream with
- *          one workspace compose the full solve, so each phase can be timed.
+ *          one workspace compose the full solve, so each phase can be timed;
+ *   bits 16-19 / 20-23  workgroup size / resident workgroup count of the
+ *          workgroup kernels; bits 24-25 tridiagonalisation shape of the
+ *          n > 64 split path; 0x4000000 (n <= 256) its full-storage form
+ *          instead of the default lower-triangle form.
  */
 #define EIGEN_GPU_BATCH_FUSED 0x100
 #define EIGEN_GPU_BATCH_SPLIT 0x200
@@ -73,6 +77,7 @@ enum eigen_GPU_batch_solver {
 #define EIGEN_GPU_BATCH_PHASE_TRI 0x1000
 #define EIGEN_GPU_BATCH_PHASE_QL 0x2000
 #define EIGEN_GPU_BATCH_PHASE_APPLY 0x4000
+#define EIGEN_GPU_BATCH_TRI_FULL 0x4000000
 
 /* Workspace size in bytes for (L, nm, n, m, solver); 0 for invalid arguments. */
 size_t eigen_GPU_batch_bufsize(int L, int nm, int n, int m, int dtype_bytes, int solver);

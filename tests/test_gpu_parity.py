@@ -152,14 +152,17 @@ def test_large_n_vs_oracle(n, dtype):
 LARGE_SPLIT_SIZES = [65, 96, 129, 160, 200, 255, 256, 257, 300, 384, 512]
 
 
-@pytest.mark.parametrize("path", ["split", "split-fallback"])
+@pytest.mark.parametrize("path", ["split", "split-fallback", "split-full"])
 @pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", LARGE_SPLIT_SIZES)
 def test_large_split_vs_oracle(n, dtype, solver, path):
     """Orders beyond the register kernels through the split path (workgroup
-    tridiagonalisation, lane-per-matrix QL with rotation log, row-block apply)
-    and through its overflow fallback (log capacity forced to 1)."""
+    tridiagonalisation, lane-per-matrix QL with rotation log, row-block apply),
+    through its overflow fallback (log capacity forced to 1) and, for n <= 256,
+    with the full-storage tridiagonalisation instead of the lower-triangle one."""
+    if path == "split-full" and n > 256:
+        pytest.skip("the full-storage form is the only one beyond n = 256")
     print(f"[parity] large {path} solver={solver} n={n} dtype={np.dtype(dtype).name}", flush=True)
     nm = n + (n & 1)
     L = 7 if n <= 256 else 3
@@ -171,6 +174,25 @@ def test_large_split_vs_oracle(n, dtype, solver, path):
     if nm > n:
         assert np.isnan(z[:, :, n:]).all()
     check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
+
+
+@pytest.mark.parametrize("path", ["split", "split-full", "fused"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [65, 129, 256, 300])
+def test_large_upper_triangle_only(n, dtype, path):
+    """Only the upper triangle (with the diagonal) is read: NaN in the strictly
+    lower triangle changes nothing, bit for bit."""
+    print(f"[parity] upper-only {path} n={n} dtype={np.dtype(dtype).name}", flush=True)
+    L = 5
+    a0 = wl.sym_rand(L, n, n, dtype=dtype, seed=7100 + n)
+    a1 = a0.copy()
+    r, c = np.tril_indices(n, -1)              # A(r, c), r > c  <->  a[k, c, r]
+    a1[:, c, r] = np.nan
+    z0, w0, i0 = run_gpu(a0, path=path)
+    z1, w1, i1 = run_gpu(a1, path=path)
+    assert i0.max() == 0 and i1.max() == 0
+    np.testing.assert_array_equal(w1, w0)
+    np.testing.assert_array_equal(z1, z0)
 
 
 @pytest.mark.parametrize("dtype", [np.float64])
