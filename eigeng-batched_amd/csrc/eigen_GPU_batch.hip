@@ -18,6 +18,7 @@
 #include "eig_uneven.hpp"
 #include "eig_split_large.hpp"
 #include "eig_check.hpp"
+#include "eig_symdmt.hpp"
 
 #include <stdio.h>
 #include <vector>
@@ -634,6 +635,54 @@ void check_entry(int L, int nm, int n, int m, T *a, T *w, T *z, hipStream_t s)
   }
 }
 
+// ---- products around the solver (MFMA) --------------------------------------
+
+// threads per workgroup: one wave per 32 x 32 output block of the triangle, at most 4
+int sd_threads(int n)
+{
+  const int nb = (n + 31) >> 5, nblk = nb * (nb + 1) / 2;
+  return 64 * (nblk < 4 ? nblk : 4);
+}
+
+bool overlap(const void *a, size_t na, const void *b, size_t nb)
+{
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + nb && y < x + na;
+}
+
+template <typename T>
+hipError_t run_symdmt(int L, int nm, int n, const T *z, const T *d, int fn, T p, T scale, T *c, hipStream_t s)
+{
+  if (L < 0 || n < 1 || n > GL_NMAX || nm < n || !z || !d || !c) return hipErrorInvalidValue;
+  if (L == 0) return hipSuccess;
+  const size_t bytes = (size_t)L * nm * n * sizeof(T);
+  if (overlap(z, bytes, c, bytes)) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)(L < 4096 ? L : 4096);
+  hipLaunchKernelGGL((eig_symdmt_kernel<T>), dim3(grid), dim3(sd_threads(n)), 0, s, L, nm, n, z, d, fn, p, scale,
+                     c);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t run_gram(int L, int nm, int n, int np, const T *y, int ldy, const T *rinv, int rstride, T alpha, T *a,
+                    hipStream_t s)
+{
+  if (L < 0 || n < 1 || n > GL_NMAX || nm < n || np < 1 || ldy < np || rstride < 0 || !y || !a)
+    return hipErrorInvalidValue;
+  if (L == 0) return hipSuccess;
+  if (overlap(y, (size_t)L * ldy * n * sizeof(T), a, (size_t)L * nm * n * sizeof(T))) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)(L < 4096 ? L : 4096);
+  const bool vec = (ldy % 4 == 0) && (np % 4 == 0) && ((uintptr_t)y % 16 == 0) &&
+                   (!rinv || ((uintptr_t)rinv % 16 == 0 && rstride % 4 == 0));
+  if (vec)
+    hipLaunchKernelGGL((eig_gram_kernel<T, true>), dim3(grid), dim3(sd_threads(n)), 0, s, L, nm, n, np, y, ldy, rinv,
+                       rstride, alpha, a);
+  else
+    hipLaunchKernelGGL((eig_gram_kernel<T, false>), dim3(grid), dim3(sd_threads(n)), 0, s, L, nm, n, np, y, ldy,
+                       rinv, rstride, alpha, a);
+  return hipGetLastError();
+}
+
 template <typename T>
 void abi_entry(int L, int nm, int n, int m, T *a, T *w, T *wk, hipStream_t s)
 {
@@ -753,6 +802,41 @@ void eigen_GPU_batch_tune_reset(void)
 {
   std::lock_guard<std::mutex> lk(tune_mu);
   tune_table.clear();
+}
+
+int eigen_GPU_batch_symdmt_FP(int L, int nm, int n, const float *z, const float *d, float *c, hipStream_t stream)
+{
+  return (int)run_symdmt<float>(L, nm, n, z, d, 0, 0.0f, 1.0f, c, stream);
+}
+
+int eigen_GPU_batch_symdmt_DP(int L, int nm, int n, const double *z, const double *d, double *c,
+                              hipStream_t stream)
+{
+  return (int)run_symdmt<double>(L, nm, n, z, d, 0, 0.0, 1.0, c, stream);
+}
+
+int eigen_GPU_batch_matpow_FP(int L, int nm, int n, const float *z, const float *w, float p, float scale, float *c,
+                              hipStream_t stream)
+{
+  return (int)run_symdmt<float>(L, nm, n, z, w, 1, p, scale, c, stream);
+}
+
+int eigen_GPU_batch_matpow_DP(int L, int nm, int n, const double *z, const double *w, double p, double scale,
+                              double *c, hipStream_t stream)
+{
+  return (int)run_symdmt<double>(L, nm, n, z, w, 1, p, scale, c, stream);
+}
+
+int eigen_GPU_batch_gram_FP(int L, int nm, int n, int p, const float *y, int ldy, const float *rinv, int rstride,
+                            float alpha, float *a, hipStream_t stream)
+{
+  return (int)run_gram<float>(L, nm, n, p, y, ldy, rinv, rstride, alpha, a, stream);
+}
+
+int eigen_GPU_batch_gram_DP(int L, int nm, int n, int p, const double *y, int ldy, const double *rinv, int rstride,
+                            double alpha, double *a, hipStream_t stream)
+{
+  return (int)run_gram<double>(L, nm, n, p, y, ldy, rinv, rstride, alpha, a, stream);
 }
 
 const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.1"; }

@@ -70,6 +70,17 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.argtypes = [ci, ci, ci, vp, vp, vp, vp, vp, ci, vp, vp, ci, vp, vp, vp]
             f.restype = ci
+        cf, cd = ctypes.c_float, ctypes.c_double
+        for name, ct in (("FP", cf), ("DP", cd)):
+            f = getattr(L, "eigen_GPU_batch_symdmt_" + name)
+            f.argtypes = [ci, ci, ci, vp, vp, vp, vp]
+            f.restype = ci
+            f = getattr(L, "eigen_GPU_batch_matpow_" + name)
+            f.argtypes = [ci, ci, ci, vp, vp, ct, ct, vp, vp]
+            f.restype = ci
+            f = getattr(L, "eigen_GPU_batch_gram_" + name)
+            f.argtypes = [ci, ci, ci, ci, vp, ci, vp, ci, ct, vp, vp]
+            f.restype = ci
         L.eigen_GPU_batch_tune_bufsize.argtypes = [ci, ci, ci, ci, ci]
         L.eigen_GPU_batch_tune_bufsize.restype = sz
         L.eigen_GPU_batch_tune_reset.argtypes = []
@@ -265,6 +276,66 @@ def autotune(a_src, solver: int = IMTQL2, stream=None):
     if rc != 0:
         raise RuntimeError(f"eigen_GPU_batch_tune failed: hipError {rc}")
     return best.value, best_ms.value, {cand[i]: cms[i] for i in range(nc.value)}, a, w
+
+
+def _sfx(t) -> str:
+    import torch
+    if t.dtype == torch.float32:
+        return "FP"
+    if t.dtype == torch.float64:
+        return "DP"
+    raise ValueError("float32 or float64 tensors only")
+
+
+def _check_rc(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: hipError {rc}")
+
+
+def symdmt(z, d, c=None, stream=None):
+    """C_k = Z_k diag(d_k) Z_k^T (eigen_GPU_batch_symdmt_*).  z (L, n, nm) in
+    the solver's layout (column j of matrix k is z[k, j, :n]), d (L, n).
+    Returns c (L, n, nm), exactly symmetric; rows n..nm-1 untouched."""
+    import torch
+    L, n, nm = z.shape
+    if c is None:
+        c = torch.empty_like(z)
+    if not (z.is_cuda and d.is_cuda and c.is_cuda) or d.dtype != z.dtype or c.dtype != z.dtype:
+        raise ValueError("device tensors of one float dtype")
+    f = getattr(lib(), "eigen_GPU_batch_symdmt_" + _sfx(z))
+    _check_rc(f(L, nm, n, z.data_ptr(), d.data_ptr(), c.data_ptr(), _stream_ptr(stream)), "symdmt")
+    return c
+
+
+def matpow(z, w, p: float, scale: float = 1.0, c=None, stream=None):
+    """C_k = scale * Z_k diag(w_k)^p Z_k^T (eigen_GPU_batch_matpow_*), from the
+    solver's eigenvectors z (L, n, nm) and eigenvalues w (L, n)."""
+    import torch
+    L, n, nm = z.shape
+    if c is None:
+        c = torch.empty_like(z)
+    if not (z.is_cuda and w.is_cuda and c.is_cuda) or w.dtype != z.dtype or c.dtype != z.dtype:
+        raise ValueError("device tensors of one float dtype")
+    f = getattr(lib(), "eigen_GPU_batch_matpow_" + _sfx(z))
+    _check_rc(f(L, nm, n, z.data_ptr(), w.data_ptr(), p, scale, c.data_ptr(), _stream_ptr(stream)), "matpow")
+    return c
+
+
+def gram(y, p: int, alpha: float = 0.0, rinv=None, rstride: int = 0, nm: int | None = None, a=None, stream=None):
+    """A_k = alpha I + Y_k^T diag(r_k) Y_k (eigen_GPU_batch_gram_*).  y (L, n,
+    ldy): y[k, i, o] = Y_k(o, i), o < p <= ldy.  rinv: None (R = I) or a device
+    tensor with r_k = rinv.flatten()[k * rstride:][:p].  Returns a (L, n, nm)."""
+    import torch
+    L, n, ldy = y.shape
+    nm = n if nm is None else nm
+    if a is None:
+        a = torch.empty((L, n, nm), dtype=y.dtype, device=y.device)
+    if rinv is not None and rinv.dtype != y.dtype:
+        raise ValueError("rinv must have y's dtype")
+    f = getattr(lib(), "eigen_GPU_batch_gram_" + _sfx(y))
+    _check_rc(f(L, nm, n, p, y.data_ptr(), ldy, None if rinv is None else rinv.data_ptr(), rstride, alpha,
+                a.data_ptr(), _stream_ptr(stream)), "gram")
+    return a
 
 
 def tune_reset() -> None:

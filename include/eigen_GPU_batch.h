@@ -170,6 +170,44 @@ int eigen_GPU_batch_tune_DP(int L, int nm, int n, const double *a_src, double *a
 /* Forget every tuned choice (all devices). */
 void eigen_GPU_batch_tune_reset(void);
 
+/*
+ * Products on either side of the solver (MFMA, FP32 / FP64 accumulation).
+ * All are asynchronous on stream and return hipSuccess, hipErrorInvalidValue
+ * (L < 0, n < 1, n > 512, nm < n, a null pointer, overlapping input and output)
+ * or the launch error; L == 0 is a no-op.  Outputs are column-major with
+ * leading dimension nm, matrix k at c + k nm n; rows n..nm-1 are not written;
+ * both triangles are written and are exactly symmetric.
+ *
+ * symdmt: C_k = Z_k diag(d_k) Z_k^T, Z_k at z + k nm n, d_k at d + k n.
+ *   Replaces the reference's single-matrix symdmt_batch experiment
+ *   (symdmt.cu:5-87), batched.
+ * matpow: C_k = scale Z_k diag(w_k)^p Z_k^T, i.e. a function of the matrix
+ *   from the solver's eigenpairs (README:171 "any power of the input matrix"):
+ *   p = -1 gives the inverse, p = 0.5 / -0.5 the square root and its inverse
+ *   (an eigenvalue <= 0 with a non-integer p gives NaN, as pow does).
+ */
+int eigen_GPU_batch_symdmt_FP(int L, int nm, int n, const float *z, const float *d, float *c,
+                              hipStream_t stream);
+int eigen_GPU_batch_symdmt_DP(int L, int nm, int n, const double *z, const double *d, double *c,
+                              hipStream_t stream);
+int eigen_GPU_batch_matpow_FP(int L, int nm, int n, const float *z, const float *w, float p,
+                              float scale, float *c, hipStream_t stream);
+int eigen_GPU_batch_matpow_DP(int L, int nm, int n, const double *z, const double *w, double p,
+                              double scale, double *c, hipStream_t stream);
+/*
+ * gram: A_k = alpha I + Y_k^T diag(r_k) Y_k, the LETKF matrix
+ *   ((k-1)/rho) I + Y^T R^-1 Y formed on the GPU (the reference forms its test
+ *   matrices on the host, main.cpp:63-195; no reference symbol).  Y_k: p x n,
+ *   column-major, leading dimension ldy >= p, at y + k ldy n.  r_k = rinv +
+ *   k rstride (rinv NULL: R = I; rstride 0: one vector for the whole batch).
+ */
+int eigen_GPU_batch_gram_FP(int L, int nm, int n, int p, const float *y, int ldy,
+                            const float *rinv, int rstride, float alpha, float *a,
+                            hipStream_t stream);
+int eigen_GPU_batch_gram_DP(int L, int nm, int n, int p, const double *y, int ldy,
+                            const double *rinv, int rstride, double alpha, double *a,
+                            hipStream_t stream);
+
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);
 
