@@ -293,6 +293,81 @@ def bench_letkf48(steps, warmup):
             "achieved_tflops": L * (10.0 / 3.0) * n ** 3 / (ms * 1e-3) / 1e12}
 
 
+def bench_letkf_step(steps, warmup, L=16384, k=64, p=400, rho=1.1, sigma=0.1, seed=0xE16E0012):
+    """The whole LETKF analysis step on the GPU for L grid points: ingest
+    A = (k-1)/rho I + Y^T Y (gram, MFMA), eigenpairs (the solver), consumer
+    T = sqrt(k-1) A^-1/2 (matpow, MFMA).  Y ~ N(0, sigma^2), p x k per point."""
+    import numpy as np
+    import torch
+    import eigeng_batched as eg
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    y = torch.randn((L, k, p), generator=g, device="cuda", dtype=torch.float32) * sigma   # y[i, c, o] = Y(o, c)
+    a = torch.empty((L, k, k), dtype=torch.float32, device="cuda")
+    t = torch.empty_like(a)
+    w = torch.empty((L, k), dtype=torch.float32, device="cuda")
+    info = torch.empty(L, dtype=torch.int32, device="cuda")
+    ws = torch.empty(eg.bufsize(L, k, k, k, 4), dtype=torch.uint8, device="cuda")
+    alpha, scale = (k - 1) / rho, float(np.sqrt(k - 1))
+
+    def step():
+        eg.gram(y, p, alpha, a=a)
+        e1.record()
+        eg.eigen_batch(a, w=w, info=info, workspace=ws)
+        e2.record()
+        eg.matpow(a, w, -0.5, scale, c=t)
+
+    for _ in range(warmup):
+        e1, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        step()
+    torch.cuda.synchronize()
+    tot = {"ingest": 0.0, "solve": 0.0, "consumer": 0.0}
+    for _ in range(steps):
+        e0, e1, e2, e3 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+        e0.record()
+        step()
+        e3.record()
+        torch.cuda.synchronize()
+        tot["ingest"] += e0.elapsed_time(e1)
+        tot["solve"] += e1.elapsed_time(e2)
+        tot["consumer"] += e2.elapsed_time(e3)
+    ph = {kk: v / steps for kk, v in tot.items()}
+    ms = sum(ph.values())
+    # accuracy of T on a sample against FP64 numpy
+    idx = np.linspace(0, L - 1, 64).astype(int)
+    yh = y[idx].cpu().numpy().astype(np.float64)
+    th = t[idx].cpu().numpy().astype(np.float64)
+    worst = 0.0
+    for j in range(len(idx)):
+        Y = yh[j].T
+        A = alpha * np.eye(k) + Y.T @ Y
+        lam, V = np.linalg.eigh(A)
+        Tref = scale * (V / np.sqrt(lam)) @ V.T
+        worst = max(worst, float(np.linalg.norm(th[j].T - Tref) / np.linalg.norm(Tref)))
+    bad = int((info != 0).sum().item())
+    tri = k * (k + 1) / 2
+    ing_bytes = 4.0 * (p * k + k * k)          # read Y, write A (both triangles)
+    ing_flops = 2.0 * p * tri                  # the triangle of Y^T Y
+    con_bytes = 4.0 * (k * k + k + k * k)      # read Z and w, write T
+    con_flops = 2.0 * k * tri                  # the triangle of Z D Z^T
+    del y, a, t, w, ws
+    torch.cuda.empty_cache()
+
+    def roof(bytes_, flops, ms_):
+        gbs = L * bytes_ / (ms_ * 1e-3) / 1e9
+        tfs = L * flops / (ms_ * 1e-3) / 1e12
+        return {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": gbs / PEAK_HBM_GBS, "bytes_per_matrix": bytes_, "mfma_tflops": tfs,
+                "mfma_frac_fp32": tfs / PEAK_FP32_TFLOPS}
+
+    return {"workload": f"LETKF analysis step on the GPU: L={L} points, k={k} members, p={p} observations, "
+                        f"rho={rho}, FP32: gram ingest + eigensolve + sqrt(k-1) A^-1/2 consumer",
+            "value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms, "phase_ms": ph,
+            "nonconverged": bad, "max_rel_err_T_vs_fp64": worst,
+            "ingest_roofline": roof(ing_bytes, ing_flops, ph["ingest"]),
+            "consumer_roofline": roof(con_bytes, con_flops, ph["consumer"])}
+
+
 def _traffic(name):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
     of known size) from profiles/, or None."""
@@ -394,6 +469,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch secondary line")
     ap.add_argument("--no-letkf48", action="store_true", help="skip the L=65536 n=48 line")
+    ap.add_argument("--no-letkf-step", action="store_true", help="skip the LETKF analysis-step line")
+    ap.add_argument("--letkf-step-only", action="store_true", help="print only the LETKF analysis-step line")
     ap.add_argument("--n256-only", action="store_true",
                     help="run only the n=256 FP64 line (for profiling its kernels)")
     ap.add_argument("--uneven-count", type=int, default=100000)
@@ -411,6 +488,9 @@ def main():
 
     if args.n256_only:
         print(json.dumps({"n256": bench_n256(args.steps, args.warmup)}))
+        return
+    if args.letkf_step_only:
+        print(json.dumps({"letkf_step": bench_letkf_step(args.steps, args.warmup)}))
         return
     dt = torch.float32 if args.dtype == "fp32" else torch.float64
     L, n = args.L, args.n
@@ -482,6 +562,8 @@ def main():
         out["n256"] = bench_n256(3, 1)
     if not args.no_letkf48:
         out["letkf48"] = bench_letkf48(5, 1)
+    if not args.no_letkf_step:
+        out["letkf_step"] = bench_letkf_step(10, 2)
     if not args.no_uneven:
         del a, a0
         torch.cuda.empty_cache()

@@ -223,4 +223,141 @@ __global__ __launch_bounds__(SD_MAX_THREADS) void eig_gram_kernel(int L, int nm,
   }
 }
 
+// ---- n <= 64: one wave per matrix, operands held in registers ----------------
+// The 16-row tiles t = 0..NT-1 of the matrix; the wave forms the tile pairs
+// (ti, tj), ti <= tj, so each operand element is loaded once per matrix.
+template <typename T, int NT>
+__device__ __forceinline__ void store_sym_tiles(T *__restrict__ O, int nm, int n, int m, int g,
+                                                const typename Mfma16<T>::v4 (&acc)[NT * (NT + 1) / 2], T diag)
+{
+  int idx = 0;
+#pragma unroll
+  for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+    for (int tj = ti; tj < NT; ++tj, ++idx)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * ti + Mfma16<T>::row(g, r);
+        const int j = 16 * tj + m;
+        if (i < n && j < n && i <= j) {
+          const T v = acc[idx][r] + ((i == j) ? diag : T(0));
+          O[j + (size_t)i * nm] = v;
+          O[i + (size_t)j * nm] = v;
+        }
+      }
+}
+
+// occupancy floor for the register-resident kernels (waves per SIMD)
+template <typename T> struct SdWaves { static constexpr int value = sizeof(T) == 8 ? 2 : 3; };
+
+template <typename T, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SdWaves<T>::value))) void eig_symdmt_wave_kernel(int L, int nm, int n, const T *__restrict__ z,
+                                                              const T *__restrict__ dsrc, int fn, T p, T scale,
+                                                              T *__restrict__ out)
+{
+  typedef typename Mfma16<T>::v4 v4;
+  constexpr int KS = 4 * NT, NP = NT * (NT + 1) / 2;     // k-steps, tile pairs
+  constexpr int KC = (sizeof(T) == 8 && NT >= 3) ? 8 : KS;
+  const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long mat = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); mat < L; mat += nw) {
+    // lane indices made opaque per matrix: otherwise every lane-constant
+    // address and mask is hoisted out of this loop into its own register
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int m = lane & 15, g = lane >> 4;
+    const T *__restrict__ Z = z + (size_t)mat * nm * n;
+    const T *__restrict__ D = dsrc + (size_t)mat * n;
+    int rows[NT];                                           // operand rows, clamped into the matrix
+#pragma unroll
+    for (int t = 0; t < NT; ++t) rows[t] = min(16 * t + m, n - 1);
+    v4 acc[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] = v4{0, 0, 0, 0};
+    // KC k-steps at a time (all of them for FP32; FP64 keeps half the operands)
+#pragma unroll 1
+    for (int k0 = 0; k0 < KS; k0 += KC) {
+      T R[NT][KC], dk[KC];
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk) {
+        const int kc = min(4 * (k0 + kk) + g, n - 1);         // clamped column of the step
+        const T *__restrict__ Zk = Z + (size_t)kc * nm;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) R[t][kk] = Zk[rows[t]];
+        dk[kk] = D[kc];
+      }
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk)
+        dk[kk] = (4 * (k0 + kk) + g < n) ? dfun(dk[kk], fn, p, scale) : T(0);
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk) {
+        // (steps beyond n carry dk = 0 and add nothing)
+#pragma unroll
+        for (int ti = 0, idx = 0; ti < NT; ++ti) {
+          const T a = R[ti][kk] * dk[kk];
+#pragma unroll
+          for (int tj = ti; tj < NT; ++tj, ++idx) acc[idx] = Mfma16<T>::op(a, R[tj][kk], acc[idx]);
+        }
+      }
+    }
+    store_sym_tiles<T, NT>(out + (size_t)mat * nm * n, nm, n, m, g, acc, T(0));
+  }
+}
+
+template <typename T, int NT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SdWaves<T>::value))) void eig_gram_wave_kernel(int L, int nm, int n, int np, const T *__restrict__ y,
+                                                            int ldy, const T *__restrict__ rinv, int rstride,
+                                                            T alpha, T *__restrict__ out)
+{
+  typedef typename Mfma16<T>::v4 v4;
+  constexpr int NP = NT * (NT + 1) / 2;
+  const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long mat = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); mat < L; mat += nw) {
+    int lane = threadIdx.x & 63;                           // opaque per matrix (see above)
+    asm volatile("" : "+v"(lane));
+    const int m = lane & 15, g = lane >> 4;
+    const T *__restrict__ Y = y + (size_t)mat * ldy * n;
+    const T *__restrict__ R = rinv ? rinv + (size_t)mat * rstride : nullptr;
+    const T *yc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) yc[t] = Y + (size_t)min(16 * t + m, n - 1) * ldy;
+    v4 acc[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] = v4{0, 0, 0, 0};
+    // software pipeline: the next step's loads are issued before this step's MFMAs
+    T x[NT][4], rv[4], xn[NT][4] = {}, rn[4] = {};
+    auto fetch = [&](int o0, T (&xo)[NT][4], T (&ro)[4]) {
+      const int o = o0 + 4 * g, cnt = np - o, oc = cnt > 0 ? o : 0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) load4<T, VEC>(yc[t] + oc, cnt, xo[t]);
+      if (R) {
+        load4<T, VEC>(R + oc, cnt, ro);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ro[q] = (q < cnt) ? T(1) : T(0);
+      }
+    };
+    fetch(0, x, rv);
+    for (int o0 = 0; o0 < np; o0 += 16) {
+      if (o0 + 16 < np) fetch(o0 + 16, xn, rn);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int idx = 0;
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti) {
+          const T a = x[ti][q] * rv[q];
+#pragma unroll
+          for (int tj = ti; tj < NT; ++tj, ++idx) acc[idx] = Mfma16<T>::op(a, x[tj][q], acc[idx]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[t][q] = xn[t][q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rv[q] = rn[q];
+    }
+    store_sym_tiles<T, NT>(out + (size_t)mat * nm * n, nm, n, m, g, acc, alpha);
+  }
+}
+
 } // namespace eigmi

@@ -657,6 +657,17 @@ hipError_t run_symdmt(int L, int nm, int n, const T *z, const T *d, int fn, T p,
   if (L == 0) return hipSuccess;
   const size_t bytes = (size_t)L * nm * n * sizeof(T);
   if (overlap(z, bytes, c, bytes)) return hipErrorInvalidValue;
+  if (n <= 64) {
+    // one wave per matrix, four waves per workgroup
+    const unsigned grid = (unsigned)((L + 3) / 4 < 2048 ? (L + 3) / 4 : 2048);
+    switch ((n + 15) >> 4) {
+    case 1: hipLaunchKernelGGL((eig_symdmt_wave_kernel<T, 1>), dim3(grid), dim3(256), 0, s, L, nm, n, z, d, fn, p, scale, c); break;
+    case 2: hipLaunchKernelGGL((eig_symdmt_wave_kernel<T, 2>), dim3(grid), dim3(256), 0, s, L, nm, n, z, d, fn, p, scale, c); break;
+    case 3: hipLaunchKernelGGL((eig_symdmt_wave_kernel<T, 3>), dim3(grid), dim3(256), 0, s, L, nm, n, z, d, fn, p, scale, c); break;
+    default: hipLaunchKernelGGL((eig_symdmt_wave_kernel<T, 4>), dim3(grid), dim3(256), 0, s, L, nm, n, z, d, fn, p, scale, c);
+    }
+    return hipGetLastError();
+  }
   const unsigned grid = (unsigned)(L < 4096 ? L : 4096);
   hipLaunchKernelGGL((eig_symdmt_kernel<T>), dim3(grid), dim3(sd_threads(n)), 0, s, L, nm, n, z, d, fn, p, scale,
                      c);
@@ -674,6 +685,22 @@ hipError_t run_gram(int L, int nm, int n, int np, const T *y, int ldy, const T *
   const unsigned grid = (unsigned)(L < 4096 ? L : 4096);
   const bool vec = (ldy % 4 == 0) && (np % 4 == 0) && ((uintptr_t)y % 16 == 0) &&
                    (!rinv || ((uintptr_t)rinv % 16 == 0 && rstride % 4 == 0));
+  if (n <= 64) {
+    const unsigned gw = (unsigned)((L + 3) / 4 < 2048 ? (L + 3) / 4 : 2048);
+    const int nt = (n + 15) >> 4;
+#define GRAM_WAVE(NT, V)                                                                                          \
+  hipLaunchKernelGGL((eig_gram_wave_kernel<T, NT, V>), dim3(gw), dim3(256), 0, s, L, nm, n, np, y, ldy, rinv,     \
+                     rstride, alpha, a)
+    if (vec) {
+      if (nt == 1) GRAM_WAVE(1, true); else if (nt == 2) GRAM_WAVE(2, true);
+      else if (nt == 3) GRAM_WAVE(3, true); else GRAM_WAVE(4, true);
+    } else {
+      if (nt == 1) GRAM_WAVE(1, false); else if (nt == 2) GRAM_WAVE(2, false);
+      else if (nt == 3) GRAM_WAVE(3, false); else GRAM_WAVE(4, false);
+    }
+#undef GRAM_WAVE
+    return hipGetLastError();
+  }
   if (vec)
     hipLaunchKernelGGL((eig_gram_kernel<T, true>), dim3(grid), dim3(sd_threads(n)), 0, s, L, nm, n, np, y, ldy, rinv,
                        rstride, alpha, a);
