@@ -353,19 +353,25 @@ def bench_letkf_step(steps, warmup, L=16384, k=64, p=400, rho=1.1, sigma=0.1, se
     del y, a, t, w, ws
     torch.cuda.empty_cache()
 
-    def roof(bytes_, flops, ms_):
+    measured = {}
+    tfile = os.path.join(ROOT, "profiles", "traffic_letkf_step_n64_fp32.json")
+    if (L, k, p) == (16384, 64, 400) and os.path.exists(tfile):
+        with open(tfile) as f:
+            measured = json.load(f)["per_kernel_bytes"]
+
+    def roof(bytes_, flops, ms_, kernel):
         gbs = L * bytes_ / (ms_ * 1e-3) / 1e9
         tfs = L * flops / (ms_ * 1e-3) / 1e12
         return {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": gbs / PEAK_HBM_GBS, "bytes_per_matrix": bytes_, "mfma_tflops": tfs,
-                "mfma_frac_fp32": tfs / PEAK_FP32_TFLOPS}
+                "frac": gbs / PEAK_HBM_GBS, "traffic": measured.get(kernel), "kernel": kernel,
+                "bytes_per_matrix": bytes_, "mfma_tflops": tfs, "mfma_frac_fp32": tfs / PEAK_FP32_TFLOPS}
 
     return {"workload": f"LETKF analysis step on the GPU: L={L} points, k={k} members, p={p} observations, "
                         f"rho={rho}, FP32: gram ingest + eigensolve + sqrt(k-1) A^-1/2 consumer",
             "value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms, "phase_ms": ph,
             "nonconverged": bad, "max_rel_err_T_vs_fp64": worst,
-            "ingest_roofline": roof(ing_bytes, ing_flops, ph["ingest"]),
-            "consumer_roofline": roof(con_bytes, con_flops, ph["consumer"])}
+            "ingest_roofline": roof(ing_bytes, ing_flops, ph["ingest"], "eig_gram_wave_kernel<float,4,true>"),
+            "consumer_roofline": roof(con_bytes, con_flops, ph["consumer"], "eig_symdmt_wave_kernel<float,4>")}
 
 
 def _traffic(name):
