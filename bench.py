@@ -374,6 +374,64 @@ def bench_letkf_step(steps, warmup, L=16384, k=64, p=400, rho=1.1, sigma=0.1, se
             "consumer_roofline": roof(con_bytes, con_flops, ph["consumer"], "eig_symdmt_wave_kernel<float,4>")}
 
 
+def vendor_baseline(a0, reps=2, max_seconds=3.0):
+    """The vendor library on the same inputs: hipSOLVER's syevjBatched (Jacobi,
+    rocSOLVER underneath), as the reference's own comparator does
+    (hipsolver.hpp:13-143, main.cpp:417-430).  Timed with HIP events like the
+    solver; its eigenpairs are checked with the same GPU checker."""
+    import ctypes
+    import torch
+    import eigeng_batched as eg
+    try:
+        hs = ctypes.CDLL("libhipsolver.so")
+    except OSError:
+        try:
+            hs = ctypes.CDLL("/opt/rocm/lib/libhipsolver.so")
+        except OSError as e:
+            return {"error": f"libhipsolver not loadable: {e}"}
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    L, n, nm = a0.shape
+    tag = "S" if a0.dtype == torch.float32 else "D"
+    f = getattr(hs, f"hipsolver{tag}syevjBatched")
+    fb = getattr(hs, f"hipsolver{tag}syevjBatched_bufferSize")
+    f.argtypes = [vp, ci, ci, ci, vp, ci, vp, vp, ci, vp, vp, ci]
+    fb.argtypes = [vp, ci, ci, ci, vp, ci, vp, ctypes.POINTER(ci), vp, ci]
+    f.restype = fb.restype = ci
+    handle, params = vp(), vp()
+    if hs.hipsolverCreate(ctypes.byref(handle)) != 0 or hs.hipsolverCreateSyevjInfo(ctypes.byref(params)) != 0:
+        return {"error": "hipsolverCreate failed"}
+    stream = torch.cuda.current_stream()
+    hs.hipsolverSetStream(handle, vp(stream.cuda_stream))
+    EIG_MODE_VECTOR, FILL_UPPER = 202, 121
+    a = a0.clone()
+    w = torch.empty((L, n), dtype=a0.dtype, device="cuda")
+    info = torch.zeros(L, dtype=torch.int32, device="cuda")
+    lwork = ci(0)
+    rc = fb(handle, EIG_MODE_VECTOR, FILL_UPPER, n, vp(a.data_ptr()), nm, vp(w.data_ptr()), ctypes.byref(lwork),
+            params, L)
+    work = torch.empty(max(lwork.value, 1), dtype=a0.dtype, device="cuda")
+    best = None
+    for _ in range(reps):
+        a.copy_(a0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = f(handle, EIG_MODE_VECTOR, FILL_UPPER, n, vp(a.data_ptr()), nm, vp(w.data_ptr()), vp(work.data_ptr()),
+               lwork.value, vp(info.data_ptr()), params, L)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+        if rc != 0 or ms > max_seconds * 1e3:
+            break
+    acc = accuracy_all(a0, a, w) if rc == 0 else None
+    hs.hipsolverDestroySyevjInfo(params)
+    hs.hipsolverDestroy(handle)
+    return {"library": f"hipSOLVER hipsolver{tag}syevjBatched (default tolerance and sweeps)", "status": rc,
+            "value": L / (best * 1e-3), "unit": "matrices/s", "ms_per_call": best, "matrices": L,
+            "nonconverged": int((info != 0).sum().item()), "accuracy_all": acc}
+
+
 def _traffic(name):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy
     of known size) from profiles/, or None."""
@@ -384,7 +442,7 @@ def _traffic(name):
         return json.load(f).get("bytes_per_launch")
 
 
-def bench_n256(steps, warmup):
+def bench_n256(steps, warmup, vendor=True):
     """L = 8192, n = 256, FP64 line: whole-call time, per-phase times, roofline,
     accuracy of every matrix."""
     import torch
@@ -395,6 +453,7 @@ def bench_n256(steps, warmup):
     bad = int((info2 != 0).sum().item())
     del a2, w2
     torch.cuda.empty_cache()
+    vend = vendor_baseline(q0[:512].contiguous(), reps=1) if vendor else None
     ph2, a2, w2, _ = time_phases(q0, 2, 1)
     del q0, a2, w2
     torch.cuda.empty_cache()
@@ -403,7 +462,7 @@ def bench_n256(steps, warmup):
     b2 = 8 * (n2 * (n2 + 1) / 2 + n2 * n2 + n2)
     return {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
             "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
-            "accuracy_all": acc2,
+            "accuracy_all": acc2, "vendor_baseline_512": vend,
             "roofline": {"bound": "mfma", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
@@ -476,6 +535,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch secondary line")
     ap.add_argument("--no-letkf48", action="store_true", help="skip the L=65536 n=48 line")
     ap.add_argument("--no-letkf-step", action="store_true", help="skip the LETKF analysis-step line")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the hipSOLVER syevjBatched comparison")
     ap.add_argument("--letkf-step-only", action="store_true", help="print only the LETKF analysis-step line")
     ap.add_argument("--n256-only", action="store_true",
                     help="run only the n=256 FP64 line (for profiling its kernels)")
@@ -493,7 +553,7 @@ def main():
     import eigeng_batched as eg
 
     if args.n256_only:
-        print(json.dumps({"n256": bench_n256(args.steps, args.warmup)}))
+        print(json.dumps({"n256": bench_n256(args.steps, args.warmup, vendor=not args.no_vendor)}))
         return
     if args.letkf_step_only:
         print(json.dumps({"letkf_step": bench_letkf_step(args.steps, args.warmup)}))
@@ -564,8 +624,10 @@ def main():
         "accuracy": acc,
     }
     out["accuracy_all"] = accuracy_all(a0, a, w)
+    if not args.no_vendor:
+        out["vendor_baseline"] = vendor_baseline(a0)
     if not args.no_n256:
-        out["n256"] = bench_n256(3, 1)
+        out["n256"] = bench_n256(3, 1, vendor=not args.no_vendor)
     if not args.no_letkf48:
         out["letkf48"] = bench_letkf48(5, 1)
     if not args.no_letkf_step:
