@@ -31,6 +31,30 @@ namespace eigmi {
 __host__ __device__ constexpr int rot_cap_of(int n) { return (3 * n * n) / 2 + 16; }
 __host__ __device__ constexpr int sw_cap_of(int n) { return 3 * n + 8; }
 
+// Indirection of the uneven-batch split route.  A class's matrices are the
+// entries [begin, end) = ctl[0..1] of the sorted list (matrix indices, -1 in
+// alignment gaps); chunk k takes the entries begin + chunk0 + idx, idx < cap,
+// and solves entry idx in workspace slot idx (slots sized for order nmx).  The
+// route runs only for a class of at least min_count entries (the fused list
+// kernel takes smaller classes), which the kernels read on the device.
+struct UvSplitArgs {
+  const int *ctl;
+  const int *list;
+  const int *nv;
+  const long long *aoff, *woff;
+  int chunk0, cap, min_count, nmx;
+};
+
+__device__ __forceinline__ bool uv_split_entry(const UvSplitArgs &u, long long idx, int &mat)
+{
+  const int b = u.ctl[0], e = u.ctl[1];
+  if (e - b < u.min_count || idx >= u.cap) return false;
+  const long long pos = (long long)b + u.chunk0 + idx;
+  if (pos >= e) return false;
+  mat = u.list[pos];
+  return mat >= 0;
+}
+
 // sweep code: l | m << 10 | lowest pair << 20  (n <= 1024)
 __device__ __forceinline__ int sw_code(int l, int m, int ilo) { return l | (m << 10) | (ilo << 20); }
 __device__ __forceinline__ int sw_m(int code) { return (code >> 10) & 1023; }
@@ -63,22 +87,42 @@ __global__ __launch_bounds__(64) void eig_tri_kernel(int L, int nm, int n, const
 
 // NLANE matrices per 64-lane workgroup (the rest of the lanes idle): the LDS
 // holds 2 NMAX NLANE values, so large orders use fewer lanes per wave.
-template <typename T, int NMAX, int NLANE = 64>
+// LIST: the uneven-batch form (u): lane idx takes list entry idx of the chunk,
+// its own order n from nv, d / e at rows i / nmx + i of de, eigenvalues at
+// w + woff, output slots at slot + idx nmx; otherwise matrix idx of a uniform
+// batch of order n.
+template <typename T, int NMAX, int NLANE = 64, bool LIST = false>
 __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
     int L, int n, const T *__restrict__ de, int ld, typename vec2<T>::type *__restrict__ rotlog,
     int rot_cap, int *__restrict__ swlog, int sw_cap, int *__restrict__ nsw_out, int *__restrict__ status,
-    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int solver)
+    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int solver, UvSplitArgs u = UvSplitArgs{})
 {
   typedef typename vec2<T>::type T2;
   __shared__ T dS[NMAX][NLANE];
   __shared__ T eS[NMAX][NLANE];
   const int lane = threadIdx.x & 63;
   if (lane >= NLANE) return;
-  const long long mat = (long long)blockIdx.x * NLANE + lane;
-  if (mat >= L) return;
+  const long long mat = (long long)blockIdx.x * NLANE + lane;   // the slot (LIST) or the matrix
+  long long gmat = mat;                                         // the matrix (info)
+  int eoff = n;                                                 // row of e(0) in de
+  T *__restrict__ wout;
+  int *__restrict__ sout;
+  if constexpr (LIST) {
+    int mi;
+    if (!uv_split_entry(u, mat, mi)) return;
+    gmat = mi;
+    n = u.nv[mi];
+    eoff = u.nmx;
+    wout = w + u.woff[mi];
+    sout = slot + (size_t)mat * u.nmx;
+  } else {
+    if (mat >= L) return;
+    wout = w + (size_t)mat * n;
+    sout = slot + (size_t)mat * n;
+  }
   for (int i = 0; i < n; ++i) {
     dS[i][lane] = de[(size_t)i * ld + mat];
-    eS[i][lane] = de[(size_t)(n + i) * ld + mat];
+    eS[i][lane] = de[(size_t)(eoff + i) * ld + mat];
   }
   T2 *__restrict__ lg = rotlog + (size_t)mat * rot_cap;
   int *__restrict__ sw = swlog + (size_t)mat * sw_cap;
@@ -180,9 +224,9 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
   status[mat] = st;
   nsw_out[mat] = nsw;
   if (st == ST_OVERFLOW) return;                 // solved again by the fused kernel
-  if (info) info[mat] = st;
+  if (info) info[gmat] = st;
   if (st != 0) {
-    for (int i = 0; i < n; ++i) w[(size_t)mat * n + i] = dS[i][lane];
+    for (int i = 0; i < n; ++i) wout[i] = dS[i][lane];
     return;
   }
   // selection sort with the reference's rule; eS now holds the positions
@@ -203,8 +247,8 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
     }
   }
   for (int i = 0; i < n; ++i) {
-    w[(size_t)mat * n + i] = dS[i][lane];
-    slot[(size_t)mat * n + (int)eS[i][lane]] = i;
+    wout[i] = dS[i][lane];
+    sout[(int)eS[i][lane]] = i;
   }
 }
 

@@ -492,6 +492,22 @@ __device__ void blocked_tridiag_sym(const T *__restrict__ Ain, int nma, T *__res
   __syncthreads();
 }
 
+template <typename T, int NT, int NBP, int NMAXT, int RG, bool SYM>
+__device__ __forceinline__ void tri_large_one(const T *__restrict__ A, int nm, int n, T *__restrict__ Wm,
+                                              TriBlkLds<T, NBP, NMAXT, RG> &S,
+                                              T (&pw)[SYM ? NT / 64 : 1][SYM ? NMAXT : 1])
+{
+  const int t = threadIdx.x;
+  if constexpr (SYM) {
+    blocked_tridiag_sym<T, NT, NBP, NMAXT, RG>(A, nm, Wm, n, n, S, pw);
+  } else {
+    for (int c = 0; c < n; ++c)
+      for (int r = t; r <= c; r += NT) Wm[r + (size_t)c * n] = A[r + (size_t)c * nm];
+    __syncthreads();
+    blocked_tridiag<T, NT, NBP, NMAXT, RG>(Wm, n, n, S);
+  }
+}
+
 template <typename T, int NT, int NBP, int NMAXT, int RG, bool SYM = false>
 __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n, const T *__restrict__ a,
                                                            T *__restrict__ W, T *__restrict__ de, int ld,
@@ -503,18 +519,37 @@ __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n,
   for (long long mat = blockIdx.x; mat < L; mat += gridDim.x) {
     const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
     T *__restrict__ Wm = W + (size_t)mat * (size_t)n * (size_t)n;
-    if constexpr (SYM) {
-      blocked_tridiag_sym<T, NT, NBP, NMAXT, RG>(A, nm, Wm, n, n, S, pw);
-    } else {
-      for (int c = 0; c < n; ++c)
-        for (int r = t; r <= c; r += NT) Wm[r + (size_t)c * n] = A[r + (size_t)c * nm];
-      __syncthreads();
-      blocked_tridiag<T, NT, NBP, NMAXT, RG>(Wm, n, n, S);
-    }
+    tri_large_one<T, NT, NBP, NMAXT, RG, SYM>(A, nm, n, Wm, S, pw);
     for (int i = t; i < n; i += NT) {
       de[(size_t)i * ld + mat] = S.d[i];
       de[(size_t)(n + i) * ld + mat] = S.e[i];
       bws[(size_t)mat * n + i] = S.beta[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Uneven-batch form: slot idx holds list entry idx of the chunk (order n from
+// nv, packed input at a + aoff, W slot nmx^2, d / e at rows i / nmx + i of de).
+template <typename T, int NT, int NBP, int NMAXT, int RG, bool SYM = true>
+__global__ __launch_bounds__(NT) void eig_tri_large_list_kernel(UvSplitArgs u, const T *__restrict__ a,
+                                                                T *__restrict__ W, T *__restrict__ de,
+                                                                T *__restrict__ bws)
+{
+  __shared__ TriBlkLds<T, NBP, NMAXT, RG> S;
+  __shared__ T pw[SYM ? NT / 64 : 1][SYM ? NMAXT : 1];
+  const int t = threadIdx.x;
+  const int ld = u.cap, nmx = u.nmx;
+  for (long long idx = blockIdx.x; idx < u.cap; idx += gridDim.x) {
+    int mat;
+    if (!uv_split_entry(u, idx, mat)) continue;       // the same for the whole workgroup
+    const int n = __builtin_amdgcn_readfirstlane(u.nv[mat]);
+    T *__restrict__ Wm = W + (size_t)idx * nmx * nmx;
+    tri_large_one<T, NT, NBP, NMAXT, RG, SYM>(a + u.aoff[mat], n, n, Wm, S, pw);
+    for (int i = t; i < n; i += NT) {
+      de[(size_t)i * ld + idx] = S.d[i];
+      de[(size_t)(nmx + i) * ld + idx] = S.e[i];
+      bws[(size_t)idx * nmx + i] = S.beta[i];
     }
     __syncthreads();
   }
@@ -535,26 +570,48 @@ struct alignas(16) ApplyLargeLds {
 // chunk and the chain fit next to each other in registers)
 // FP64 is held to 128 VGPRs (four waves per SIMD: two 8-wave workgroups per
 // CU); FP32 keeps the registers its branch-free chain wants
-template <typename T> struct ApWaves { static constexpr int value = sizeof(T) == 8 ? 4 : 1; };
+// occupancy floor (waves per SIMD) of the apply kernel: FP64 is held to 128
+// VGPRs; FP32 with at most 4 waves per workgroup to 168 (3 workgroups per CU);
+// wider FP32 workgroups are left uncapped (measured best)
+template <typename T, int NCH> struct ApWaves {
+  static constexpr int value = sizeof(T) == 8 ? 4 : (NCH <= 4 ? 3 : 1);
+};
 
-template <typename T, int NCH, int CW>
-__global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWaves<T>::value))) void eig_apply_large_kernel(
+// LIST: the uneven-batch form (u), one workgroup per (slot, row block of
+// order nmx); otherwise (matrix, row block) of a uniform batch of order n.
+template <typename T, int NCH, int CW, bool LIST = false>
+__global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWaves<T, NCH>::value))) void eig_apply_large_kernel(
     int L, int nm, int n, T *__restrict__ a, const T *__restrict__ W, const T *__restrict__ bws,
     const typename vec2<T>::type *__restrict__ rotlog, int rot_cap, const int *__restrict__ swlog, int sw_cap,
-    const int *__restrict__ nsw_in, const int *__restrict__ status, const int *__restrict__ slot)
+    const int *__restrict__ nsw_in, const int *__restrict__ status, const int *__restrict__ slot,
+    UvSplitArgs u = UvSplitArgs{})
 {
   typedef typename vec2<T>::type T2;
   __shared__ ApplyLargeLds<T, NCH, CW> S;
-  const int nrb = (n + 63) >> 6;
-  const long long mat = blockIdx.x / nrb;
+  const int nrb = ((LIST ? u.nmx : n) + 63) >> 6;
+  const long long mat = blockIdx.x / nrb;            // the slot (LIST) or the matrix
   const int rb = blockIdx.x - (int)(mat * nrb);
-  if (mat >= L) return;
+  size_t wstride = (size_t)n * n, bstride = n, sstride = n;
+  T *__restrict__ A;
+  if constexpr (LIST) {
+    int mi;
+    if (!uv_split_entry(u, mat, mi)) return;
+    n = __builtin_amdgcn_readfirstlane(u.nv[mi]);
+    nm = n;
+    if (rb * 64 >= n) return;
+    wstride = (size_t)u.nmx * u.nmx;
+    bstride = sstride = u.nmx;
+    A = a + u.aoff[mi];
+  } else {
+    if (mat >= L) return;
+    A = a + (size_t)mat * (size_t)nm * (size_t)n;
+  }
   if (status[mat] != 0) return;                      // overflow: solved by the fallback
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int r = rb * 64 + lane;                      // row of Q handled by this lane
   const int c0 = q * CW;                             // first column of this wave
-  const T *__restrict__ Wm = W + (size_t)mat * (size_t)n * (size_t)n;
-  const T *__restrict__ beta = bws + (size_t)mat * n;
+  const T *__restrict__ Wm = W + (size_t)mat * wstride;
+  const T *__restrict__ beta = bws + (size_t)mat * bstride;
 
   T x[CW];
 #pragma unroll
@@ -719,9 +776,8 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
   }
 
   // ---- C. permuted store: column j goes to slot[j] --------------------------
-  if (lane < CW) S.pos[q][lane] = (c0 + lane < n) ? slot[(size_t)mat * n + c0 + lane] : 0;
+  if (lane < CW) S.pos[q][lane] = (c0 + lane < n) ? slot[(size_t)mat * sstride + c0 + lane] : 0;
   wsync();
-  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
   if (r < n) {
 #pragma unroll
     for (int j = 0; j < CW; ++j)

@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "eig_wave.hpp"
 #include "eig_global.hpp"
+#include "eig_split.hpp"
 
 namespace eigmi {
 
@@ -186,6 +187,34 @@ __global__ __launch_bounds__(GL_THREADS) void eig_global_list_kernel(
     const int mat = list[t0];
     const int n = __builtin_amdgcn_readfirstlane(nv[mat]);
     global_solve<T, GL_THREADS>(a + aoff[mat], n, n, w + woff[mat], info ? info + mat : nullptr, solver, Z, S);
+  }
+}
+
+// Split route for a size class: when the class has at least min_count
+// entries, move its work cursor to the end, so that the fused list kernel of
+// the class finds nothing left to take.
+__global__ void uv_claim_kernel(int *__restrict__ ctl, int min_count)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0 && ctl[1] - ctl[0] >= min_count) ctl[2] = ctl[1];
+}
+
+// Split route, per chunk: the entries whose rotation log overflowed (status
+// ST_OVERFLOW) are solved from their untouched input by the workgroup kernel.
+template <typename T>
+__global__ __launch_bounds__(GL_THREADS) void eig_global_fallback_list_kernel(UvSplitArgs u,
+                                                                              const int *__restrict__ status,
+                                                                              T *__restrict__ a, T *__restrict__ w,
+                                                                              int *__restrict__ info, int solver,
+                                                                              T *__restrict__ zws)
+{
+  __shared__ GlobalLds<T> S;
+  T *__restrict__ Z = zws + (size_t)blockIdx.x * u.nmx * u.nmx;
+  for (long long idx = blockIdx.x; idx < u.cap; idx += gridDim.x) {
+    int mat;
+    if (!uv_split_entry(u, idx, mat)) continue;        // the same for the whole workgroup
+    if (status[idx] != ST_OVERFLOW) continue;
+    const int n = __builtin_amdgcn_readfirstlane(u.nv[mat]);
+    global_solve<T, GL_THREADS>(a + u.aoff[mat], n, n, w + u.woff[mat], info ? info + mat : nullptr, solver, Z, S);
   }
 }
 

@@ -508,9 +508,9 @@ UvClasses uv_classes(int tbytes)
 {
   UvClasses c{};
   const int f32[] = {8, 16, 32, 64, 128, GL_NMAX};
-  const int f64[] = {8, 16, 32, 64, GL_NMAX};
+  const int f64[] = {8, 16, 32, 64, 256, GL_NMAX};
   const int *h = tbytes == 4 ? f32 : f64;
-  c.ncls = tbytes == 4 ? 6 : 5;
+  c.ncls = 6;
   for (int i = 0; i < c.ncls; ++i) {
     c.hi[i] = h[i];
     c.g[i] = h[i] < 64 ? 64 / h[i] : 1;
@@ -520,8 +520,43 @@ UvClasses uv_classes(int tbytes)
 
 int uv_global_grid(int L) { return L < 1024 ? L : 1024; }
 
+// Split route of the uneven path: the class of orders (64, 128] (FP32) or
+// (64, 256] (FP64) runs through the large split kernels (workgroup
+// tridiagonalisation, lane-per-matrix QL, row-block apply) when it holds at
+// least UV_SPLIT_MIN matrices, since the QL recurrence only pays off on many
+// matrices at once; smaller classes keep the fused kernels.  The route works
+// in chunks of cap matrices (workspace slots of order nmx).
+constexpr int UV_SPLIT_MIN_F32 = 4096, UV_SPLIT_MIN_F64 = 1024;
+int uv_split_nmx(int nmax, int tb)
+{
+  const int hi = tb == 4 ? 128 : 256;
+  return nmax > 64 ? (nmax < hi ? nmax : hi) : 0;     // 0: no split route
+}
+int uv_split_cap(int L, int tb)
+{
+  const int ch = tb == 4 ? 32768 : 8192;
+  return L < ch ? L : ch;
+}
+int uv_fallback_grid(int cap) { return cap < 256 ? cap : 256; }
+
+struct UvSplitLayout {
+  size_t wm, de, bw, rot, sw, nsw, status, slot, total;
+  UvSplitLayout(int cap, int nmx, int tb)
+  {
+    wm = 0;
+    de = wm + al256((size_t)cap * nmx * nmx * tb);
+    bw = de + al256((size_t)2 * nmx * cap * tb);
+    rot = bw + al256((size_t)cap * nmx * tb);
+    sw = rot + al256((size_t)cap * rot_cap_of(nmx) * 2 * tb);
+    nsw = sw + al256((size_t)cap * sw_cap_of(nmx) * sizeof(int));
+    status = nsw + al256((size_t)cap * sizeof(int));
+    slot = status + al256((size_t)cap * sizeof(int));
+    total = slot + al256((size_t)cap * nmx * sizeof(int));
+  }
+};
+
 struct UvLayout {
-  size_t ctl, list, aoff, woff, z, total;
+  size_t ctl, list, aoff, woff, z, split, total;
   UvLayout(int L, int nmax, int tb)
   {
     ctl = 0;
@@ -530,7 +565,11 @@ struct UvLayout {
     woff = aoff + al256((size_t)L * sizeof(long long));
     z = woff + al256((size_t)L * sizeof(long long));
     const bool glob = !use_wave_kernel(nmax, tb);
-    total = z + (glob ? al256((size_t)uv_global_grid(L) * nmax * nmax * tb) : 0);
+    const int nmx = uv_split_nmx(nmax, tb), cap = uv_split_cap(L, tb);
+    const size_t zg = glob ? (size_t)uv_global_grid(L) * nmax * nmax * tb : 0;
+    const size_t zs = nmx ? (size_t)uv_fallback_grid(cap) * nmx * nmx * tb : 0;
+    split = z + al256(zg > zs ? zg : zs);
+    total = split + (nmx ? UvSplitLayout(cap, nmx, tb).total : 0);
   }
 };
 
@@ -564,6 +603,44 @@ void launch_wave_list(int *ctl, const int *list, const int *nv, const long long 
                      a, w, info, solver);
 }
 
+// The split route's kernels for one class, chunk by chunk (chunks beyond the
+// class's entries find nothing to do; see uv_split_entry).
+template <typename T>
+void run_uneven_split(int L, int nmx, int *cc, const int *list, const int *nv, const long long *ao,
+                      const long long *wo, T *a, T *w, int *info, int solver, char *base, T *zws, hipStream_t s)
+{
+  typedef typename vec2<T>::type T2;
+  const int tb = (int)sizeof(T);
+  const int cap = uv_split_cap(L, tb);
+  const UvSplitLayout lay(cap, nmx, tb);
+  T *Wm = (T *)(base + lay.wm), *de = (T *)(base + lay.de), *bw = (T *)(base + lay.bw);
+  T2 *rot = (T2 *)(base + lay.rot);
+  int *sw = (int *)(base + lay.sw), *nsw = (int *)(base + lay.nsw);
+  int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
+  const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(nmx), scap = sw_cap_of(nmx);
+  const int sv = solver & 0xff;
+  const int min_count = (solver & FLAG_SPLIT) ? 1 : (tb == 4 ? UV_SPLIT_MIN_F32 : UV_SPLIT_MIN_F64);
+  constexpr int QMAX = sizeof(T) == 4 ? 128 : 256, NLANE = sizeof(T) == 4 ? 64 : 32;
+  const int nrb = (nmx + 63) >> 6;
+  for (long long c0 = 0; c0 < L; c0 += cap) {
+    const UvSplitArgs u{cc, list, nv, ao, wo, (int)c0, cap, min_count, nmx};
+    hipLaunchKernelGGL((eig_tri_large_list_kernel<T, 256, 8, 256, 1, true>), dim3(cap < 1024 ? cap : 1024),
+                       dim3(256), 0, s, u, (const T *)a, Wm, de, bw);
+    hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0, s,
+                       cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+    if (nmx <= 128)
+      hipLaunchKernelGGL((eig_apply_large_kernel<T, 4, 32, true>), dim3((unsigned)cap * nrb), dim3(256), 0, s, cap,
+                         0, 0, a, (const T *)Wm, (const T *)bw, (const T2 *)rot, rcap, (const int *)sw, scap,
+                         (const int *)nsw, (const int *)st, (const int *)slot, u);
+    else
+      hipLaunchKernelGGL((eig_apply_large_kernel<T, 8, 32, true>), dim3((unsigned)cap * nrb), dim3(512), 0, s, cap,
+                         0, 0, a, (const T *)Wm, (const T *)bw, (const T2 *)rot, rcap, (const int *)sw, scap,
+                         (const int *)nsw, (const int *)st, (const int *)slot, u);
+    hipLaunchKernelGGL((eig_global_fallback_list_kernel<T>), dim3(uv_fallback_grid(cap)), dim3(GL_THREADS), 0, s,
+                       u, (const int *)st, a, w, info, sv, zws);
+  }
+}
+
 template <typename T>
 hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, const long long *woff, T *a, T *w,
                       void *wk, hipStream_t s, int solver, int *info)
@@ -582,10 +659,14 @@ hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, con
   hipLaunchKernelGGL(uv_bin_kernel, dim3(1), dim3(UV_THREADS), 0, s, L, nmax, nv, aws, wws,
                      aoff == nullptr ? 1 : 0, list, ctl, info, cls);
   const long long *ao = aoff ? aoff : aws, *wo = woff ? woff : wws;
+  const int nmx = uv_split_nmx(nmax, tb);
   for (int c = 0; c < cls.ncls; ++c) {
     const int lo = c == 0 ? 0 : cls.hi[c - 1];
     if (lo >= nmax) break;
     int *cc = ctl + 4 * c;
+    const bool split = nmx && lo == 64 && !(solver & FLAG_FUSED);
+    const int min_count = (solver & FLAG_SPLIT) ? 1 : (tb == 4 ? UV_SPLIT_MIN_F32 : UV_SPLIT_MIN_F64);
+    if (split) hipLaunchKernelGGL(uv_claim_kernel, dim3(1), dim3(64), 0, s, cc, min_count);
     switch (cls.hi[c]) {
     case 8: launch_wave_list<T, 8>(cc, list, nv, ao, wo, a, w, info, base, s); break;
     case 16: launch_wave_list<T, 16>(cc, list, nv, ao, wo, a, w, info, base, s); break;
@@ -596,6 +677,7 @@ hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, con
       hipLaunchKernelGGL((eig_global_list_kernel<T>), dim3(uv_global_grid(L)), dim3(GL_THREADS), 0, s, cc,
                          list, nv, ao, wo, a, w, info, base, zws, (long long)nmax * nmax);
     }
+    if (split) run_uneven_split<T>(L, nmx, cc, list, nv, ao, wo, a, w, info, solver, ws + lay.split, zws, s);
   }
   return hipGetLastError();
 }

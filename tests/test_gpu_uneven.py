@@ -171,6 +171,52 @@ def test_uneven_graph_replay():
         assert int(info.abs().max().item()) == 0
 
 
+@pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
+@pytest.mark.parametrize("dtype,lo,hi,nmax", [(np.float32, 1, 128, 128), (np.float32, 60, 128, 140),
+                                               (np.float64, 1, 256, 256), (np.float64, 50, 200, 300)])
+@pytest.mark.parametrize("tiny", [False, True])
+def test_uneven_split_route_vs_oracle(dtype, lo, hi, nmax, solver, tiny):
+    """The split route of the class beyond order 64 (large-order split kernels
+    driven by the sorted list), forced for a small batch; with the rotation log
+    shrunk, every matrix of that class takes the route's workgroup fallback."""
+    print(f"[uneven] split route dtype={np.dtype(dtype).name} n={lo}..{hi} nmax={nmax} solver={solver} "
+          f"tiny={tiny}", flush=True)
+    ns = random_orders(90, lo, hi, seed=17 + solver + hi)
+    mats = make_mats(ns, dtype, seed=19)
+    a, nv = eg.pack_uneven([torch.from_numpy(m) for m in mats], DEV)
+    flags = eg.FLAG_SPLIT | (eg.FLAG_TINYLOG if tiny else 0)
+    a, w, info = eg.eigen_batch_uneven(a, nv, nmax, solver=solver | flags)
+    torch.cuda.synchronize()
+    assert int(info.abs().max().item()) == 0
+    res = eg.unpack_uneven(a, w, [int(n) for n in ns])
+    check_against_oracle(mats, res, dtype, solver)
+
+
+def test_uneven_split_route_by_count():
+    """Without forcing: a class beyond order 64 with at least 4096 matrices
+    (FP32) takes the split route on its own; every matrix passes the checker
+    and a sample matches the oracle."""
+    ns = np.concatenate([random_orders(4200, 65, 128, seed=31), random_orders(300, 2, 64, seed=32)])
+    ns = np.random.default_rng(33).permutation(ns)
+    mats = make_mats(ns, np.float32, seed=34)
+    a0, nv = eg.pack_uneven([torch.from_numpy(m) for m in mats], DEV)
+    a, w, info = eg.eigen_batch_uneven(a0.clone(), nv, 128)
+    torch.cuda.synchronize()
+    assert int(info.abs().max().item()) == 0
+    res = eg.unpack_uneven(a, w, [int(n) for n in ns])
+    pick = np.random.default_rng(35).choice(len(ns), size=160, replace=False)
+    check_against_oracle([mats[k] for k in pick], [res[k] for k in pick], np.float32)
+    # the fused kernels give the same eigenvalues
+    a2, w2, info2 = eg.eigen_batch_uneven(a0.clone(), nv, 128, solver=eg.IMTQL2 | eg.FLAG_FUSED)
+    torch.cuda.synchronize()
+    assert int(info2.abs().max().item()) == 0
+    wa, wb = w.cpu().numpy(), w2.cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(ns)])
+    for k in range(len(ns)):
+        x, y = wa[off[k]:off[k + 1]], wb[off[k]:off[k + 1]]
+        assert np.max(np.abs(x - y)) <= 64 * np.finfo(np.float32).eps * max(np.abs(y).max(), 1e-30)
+
+
 def test_uneven_invalid_args():
     a = torch.zeros(16, device=DEV)
     nv = torch.full((1,), 4, dtype=torch.int32, device=DEV)
