@@ -129,6 +129,25 @@ def time_solver(a0, solver, steps, warmup):
     return ms, wall, a, w, info
 
 
+def tuned_line(a0, plain_ms, steps, warmup):
+    """The autotuner on the batch (every candidate path and launch shape timed
+    once on these inputs), then the remembered winner timed like the plain
+    call, with the GPU checker on its results (equal-residual comparison)."""
+    import torch
+    import eigeng_batched as eg
+    best, best_ms, table, _, _ = eg.autotune(a0)
+    ms, _, a, w, info = time_solver(a0, 0, steps, warmup)
+    acc = accuracy_all(a0, a, w)
+    bad = int((info != 0).sum().item())
+    L = a0.shape[0]
+    del a, w
+    torch.cuda.empty_cache()
+    eg.tune_reset()
+    return {"value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms, "best_bits": hex(best),
+            "candidates_ms": {hex(k): v for k, v in table.items()}, "vs_plain": plain_ms / ms,
+            "nonconverged": bad, "accuracy_all": acc}
+
+
 def time_phases(a0, steps, warmup):
     """Per-phase HIP-event times (ms) of the split paths: the phases are issued
     as separate library calls on one stream with events between them (n <= 64:
@@ -442,7 +461,7 @@ def _traffic(name):
         return json.load(f).get("bytes_per_launch")
 
 
-def bench_n256(steps, warmup, vendor=True):
+def bench_n256(steps, warmup, vendor=True, tuned=True):
     """L = 8192, n = 256, FP64 line: whole-call time, per-phase times, roofline,
     accuracy of every matrix."""
     import torch
@@ -453,6 +472,7 @@ def bench_n256(steps, warmup, vendor=True):
     bad = int((info2 != 0).sum().item())
     del a2, w2
     torch.cuda.empty_cache()
+    tuned2 = tuned_line(q0, ms2, steps, warmup) if tuned else None
     vend = vendor_baseline(q0[:512].contiguous(), reps=1) if vendor else None
     ph2, a2, w2, _ = time_phases(q0, 2, 1)
     del q0, a2, w2
@@ -462,7 +482,7 @@ def bench_n256(steps, warmup, vendor=True):
     b2 = 8 * (n2 * (n2 + 1) / 2 + n2 * n2 + n2)
     return {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
             "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
-            "accuracy_all": acc2, "vendor_baseline_512": vend,
+            "accuracy_all": acc2, "tuned": tuned2, "vendor_baseline_512": vend,
             "roofline": {"bound": "mfma", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
@@ -536,6 +556,7 @@ def main():
     ap.add_argument("--no-letkf48", action="store_true", help="skip the L=65536 n=48 line")
     ap.add_argument("--no-letkf-step", action="store_true", help="skip the LETKF analysis-step line")
     ap.add_argument("--no-vendor", action="store_true", help="skip the hipSOLVER syevjBatched comparison")
+    ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned lines")
     ap.add_argument("--letkf-step-only", action="store_true", help="print only the LETKF analysis-step line")
     ap.add_argument("--n256-only", action="store_true",
                     help="run only the n=256 FP64 line (for profiling its kernels)")
@@ -553,7 +574,8 @@ def main():
     import eigeng_batched as eg
 
     if args.n256_only:
-        print(json.dumps({"n256": bench_n256(args.steps, args.warmup, vendor=not args.no_vendor)}))
+        print(json.dumps({"n256": bench_n256(args.steps, args.warmup, vendor=not args.no_vendor,
+                                             tuned=not args.no_tuned)}))
         return
     if args.letkf_step_only:
         print(json.dumps({"letkf_step": bench_letkf_step(args.steps, args.warmup)}))
@@ -624,10 +646,12 @@ def main():
         "accuracy": acc,
     }
     out["accuracy_all"] = accuracy_all(a0, a, w)
+    if not args.no_tuned:
+        out["tuned"] = tuned_line(a0, ms, args.steps, args.warmup)
     if not args.no_vendor:
         out["vendor_baseline"] = vendor_baseline(a0)
     if not args.no_n256:
-        out["n256"] = bench_n256(3, 1, vendor=not args.no_vendor)
+        out["n256"] = bench_n256(3, 1, vendor=not args.no_vendor, tuned=not args.no_tuned)
     if not args.no_letkf48:
         out["letkf48"] = bench_letkf48(5, 1)
     if not args.no_letkf_step:
