@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "eig_global.hpp"
 #include "eig_split.hpp"
+#include <cstddef>
 
 namespace eigmi {
 
@@ -274,7 +275,10 @@ __device__ void blocked_tridiag_sym(const T *__restrict__ Ain, int nma, T *__res
 {
   constexpr int RT = NT / RG;
   static_assert(2 * RT >= NMAXT, "two rows per thread cover the matrix");
-  static_assert(NT % 32 == 0 && NBP * NMAXT >= 32 * 33, "transpose staging fits in U");
+  // the transpose staging uses U and V, which are adjacent
+  static_assert(NT % 32 == 0 && 2 * NBP * NMAXT >= 32 * 33, "transpose staging fits in U and V");
+  typedef TriBlkLds<T, NBP, NMAXT, RG> Lds_t;
+  static_assert(offsetof(Lds_t, V) == offsetof(Lds_t, U) + sizeof(T) * NBP * NMAXT, "V follows U");
   const int t = threadIdx.x;
   const int g = t / RT;
   const int r0 = t - g * RT, r1 = r0 + RT;
@@ -508,8 +512,15 @@ __device__ __forceinline__ void tri_large_one(const T *__restrict__ A, int nm, i
   }
 }
 
+// occupancy floor of the tridiagonalisation (waves per SIMD): the FP64
+// lower-triangle form is held to 128 VGPRs (four 4-wave workgroups per CU)
+template <typename T, bool SYM> struct TriWaves { static constexpr int value = (SYM && sizeof(T) == 8) ? 4 : 1; };
+// panel width of the lower-triangle form: 4 for FP64 (the LDS of four
+// workgroups per CU), 8 for FP32
+template <typename T> struct TriNbp { static constexpr int value = sizeof(T) == 8 ? 4 : 8; };
+
 template <typename T, int NT, int NBP, int NMAXT, int RG, bool SYM = false>
-__global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n, const T *__restrict__ a,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TriWaves<T, SYM>::value))) void eig_tri_large_kernel(int L, int nm, int n, const T *__restrict__ a,
                                                            T *__restrict__ W, T *__restrict__ de, int ld,
                                                            T *__restrict__ bws)
 {
@@ -532,7 +543,7 @@ __global__ __launch_bounds__(NT) void eig_tri_large_kernel(int L, int nm, int n,
 // Uneven-batch form: slot idx holds list entry idx of the chunk (order n from
 // nv, packed input at a + aoff, W slot nmx^2, d / e at rows i / nmx + i of de).
 template <typename T, int NT, int NBP, int NMAXT, int RG, bool SYM = true>
-__global__ __launch_bounds__(NT) void eig_tri_large_list_kernel(UvSplitArgs u, const T *__restrict__ a,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TriWaves<T, SYM>::value))) void eig_tri_large_list_kernel(UvSplitArgs u, const T *__restrict__ a,
                                                                 T *__restrict__ W, T *__restrict__ de,
                                                                 T *__restrict__ bws)
 {

@@ -276,13 +276,13 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
       if (gc) grid = (unsigned)(128 * gc) < grid ? (unsigned)(128 * gc) : grid;
       if (sym) {
         if (cfg == 3)
-          hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, 8, 256, 4, true>), dim3(grid), dim3(1024), 0, s, cur,
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, TriNbp<T>::value, 256, 4, true>), dim3(grid), dim3(1024), 0, s, cur,
                              nm, n, ac, Wm, de, lc, bw);
         else if (cfg == 2)
-          hipLaunchKernelGGL((eig_tri_large_kernel<T, 512, 8, 256, 2, true>), dim3(grid), dim3(512), 0, s, cur, nm,
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 512, TriNbp<T>::value, 256, 2, true>), dim3(grid), dim3(512), 0, s, cur, nm,
                              n, ac, Wm, de, lc, bw);
         else
-          hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, 8, 256, 1, true>), dim3(grid), dim3(256), 0, s, cur, nm,
+          hipLaunchKernelGGL((eig_tri_large_kernel<T, 256, TriNbp<T>::value, 256, 1, true>), dim3(grid), dim3(256), 0, s, cur, nm,
                              n, ac, Wm, de, lc, bw);
       } else if (n <= 256) {
         if (cfg == 3)
@@ -624,7 +624,7 @@ void run_uneven_split(int L, int nmx, int *cc, const int *list, const int *nv, c
   const int nrb = (nmx + 63) >> 6;
   for (long long c0 = 0; c0 < L; c0 += cap) {
     const UvSplitArgs u{cc, list, nv, ao, wo, (int)c0, cap, min_count, nmx};
-    hipLaunchKernelGGL((eig_tri_large_list_kernel<T, 256, 8, 256, 1, true>), dim3(cap < 1024 ? cap : 1024),
+    hipLaunchKernelGGL((eig_tri_large_list_kernel<T, 256, TriNbp<T>::value, 256, 1, true>), dim3(cap < 1024 ? cap : 1024),
                        dim3(256), 0, s, u, (const T *)a, Wm, de, bw);
     hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0, s,
                        cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
