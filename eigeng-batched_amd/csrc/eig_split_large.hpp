@@ -721,9 +721,10 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
       } else {
         // FP64: 4-pair blocks from the one holding the window's top pair down to
         // the one holding its bottom pair; the rotations of a block are read
-        // together (the next block's while the current one is applied), and the
-        // identity rotations outside the window (see the FP32 form above) make
-        // every pair of a block safe to apply.
+        // together from LDS as the block starts (prefetching the next block's
+        // into a second register set cost more in moves and spills than the
+        // LDS latency it hid), and the identity rotations outside the window
+        // (see the FP32 form above) make every pair of a block safe to apply.
         if (enters || top >= bot) {
           constexpr int PB = 4;
           const int ptop = enters ? CW - 1 : top - (c0 - 1);   // highest pair index p to apply
@@ -738,18 +739,12 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
               if (j == btop * PB + PB - 1) h1 = x[j];
           }
           const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
-          T2 g[PB], gn[PB];
-#pragma unroll
-          for (int jj = 0; jj < PB; ++jj) gn[jj] = S.rs[q][btop * PB + jj];
+          T2 g[PB];
 #pragma unroll
           for (int b = CW / PB - 1; b >= 0; --b) {
             if (b <= btop && b >= bbot) {
 #pragma unroll
-              for (int jj = 0; jj < PB; ++jj) g[jj] = gn[jj];
-              if (b > bbot) {
-#pragma unroll
-                for (int jj = 0; jj < PB; ++jj) gn[jj] = S.rs[q][(b - 1) * PB + jj];
-              }
+              for (int jj = 0; jj < PB; ++jj) g[jj] = S.rs[q][b * PB + jj];
 #pragma unroll
               for (int jj = PB - 1; jj >= 0; --jj) {
                 const int p = b * PB + jj;
