@@ -581,12 +581,9 @@ struct alignas(16) ApplyLargeLds {
 // chunk and the chain fit next to each other in registers)
 // FP64 is held to 128 VGPRs (four waves per SIMD: two 8-wave workgroups per
 // CU); FP32 keeps the registers its branch-free chain wants
-// occupancy floor (waves per SIMD) of the apply kernel: FP64 is held to 128
-// VGPRs; FP32 with at most 4 waves per workgroup to 168 (3 workgroups per CU);
-// wider FP32 workgroups are left uncapped (measured best)
-template <typename T, int NCH> struct ApWaves {
-  static constexpr int value = sizeof(T) == 8 ? 4 : (NCH <= 4 ? 3 : 1);
-};
+// occupancy floor (waves per SIMD) of the apply kernel: held to 128 VGPRs, two
+// 8-wave workgroups per CU
+template <typename T, int NCH> struct ApWaves { static constexpr int value = 4; };
 
 // LIST: the uneven-batch form (u), one workgroup per (slot, row block of
 // order nmx); otherwise (matrix, row block) of a uniform batch of order n.
@@ -695,81 +692,56 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
       const int code2 = (s + 2 < nsw) ? sw[s + 2] : 0;
       pre = (s + 1 < nsw) ? fetch(code1, base + cnt) : T2{T(1), T(0)};
       wsync();
-      if constexpr (sizeof(T) == 4) {
-        if (enters || top >= bot) {
-          // Branch-free chain over all pairs of the chunk: outside [bot, top]
-          // the rotation is the identity, which leaves every column as it is
-          // while h1 carries the original value of the column above (so the
-          // window starts with h1 = x(m), and x(ilo) receives the pending value).
-          T h1 = enters ? S.carry[q][lane] : x[CW - 1];
-          const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
+      // 4-pair blocks from the one holding the window's top pair down to
+      // the one holding its bottom pair; the rotations of a block are read
+      // together from LDS as the block starts (prefetching the next block's
+      // into a second register set cost more in moves and spills than the
+      // LDS latency it hid), and the identity rotations outside the window
+      // make every pair of a block safe to apply.
+      if (enters || top >= bot) {
+        constexpr int PB = 4;
+        const int ptop = enters ? CW - 1 : top - (c0 - 1);   // highest pair index p to apply
+        const int pbot = bot - (c0 - 1);                      // lowest pair index p in the window
+        const int btop = ptop / PB, bbot = pbot / PB;
+        T h1 = T(0);
+        if (enters) {
+          h1 = S.carry[q][lane];
+        } else {
 #pragma unroll
-          for (int p = CW - 1; p >= 1; --p) {
-            const T2 g = S.rs[q][p];
-            const T h0 = x[p - 1];
-            x[p] = g.y * h0 + g.x * h1;
-            h1 = g.x * h0 - g.y * h1;
-          }
-          if (q > 0) {
-            const T2 g = S.rs[q][0];
-            x[0] = g.y * xlow + g.x * h1;
-            S.carry[q - 1][lane] = g.x * xlow - g.y * h1;  // pending (or final) column c0-1
-          } else {
-            x[0] = h1;                                      // column 0 of the matrix
-          }
+          for (int j = 0; j < CW; ++j)
+            if (j == btop * PB + PB - 1) h1 = x[j];
         }
-      } else {
-        // FP64: 4-pair blocks from the one holding the window's top pair down to
-        // the one holding its bottom pair; the rotations of a block are read
-        // together from LDS as the block starts (prefetching the next block's
-        // into a second register set cost more in moves and spills than the
-        // LDS latency it hid), and the identity rotations outside the window
-        // (see the FP32 form above) make every pair of a block safe to apply.
-        if (enters || top >= bot) {
-          constexpr int PB = 4;
-          const int ptop = enters ? CW - 1 : top - (c0 - 1);   // highest pair index p to apply
-          const int pbot = bot - (c0 - 1);                      // lowest pair index p in the window
-          const int btop = ptop / PB, bbot = pbot / PB;
-          T h1 = T(0);
-          if (enters) {
-            h1 = S.carry[q][lane];
-          } else {
+        const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
+        T2 g[PB];
 #pragma unroll
-            for (int j = 0; j < CW; ++j)
-              if (j == btop * PB + PB - 1) h1 = x[j];
-          }
-          const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
-          T2 g[PB];
+        for (int b = CW / PB - 1; b >= 0; --b) {
+          if (b <= btop && b >= bbot) {
 #pragma unroll
-          for (int b = CW / PB - 1; b >= 0; --b) {
-            if (b <= btop && b >= bbot) {
+            for (int jj = 0; jj < PB; ++jj) g[jj] = S.rs[q][b * PB + jj];
 #pragma unroll
-              for (int jj = 0; jj < PB; ++jj) g[jj] = S.rs[q][b * PB + jj];
-#pragma unroll
-              for (int jj = PB - 1; jj >= 0; --jj) {
-                const int p = b * PB + jj;
-                if (p >= 1) {
-                  const T h0 = x[p >= 1 ? p - 1 : 0];
-                  x[p] = g[jj].y * h0 + g[jj].x * h1;
-                  h1 = g[jj].x * h0 - g[jj].y * h1;
-                } else if (q > 0) {
-                  x[0] = g[jj].y * xlow + g[jj].x * h1;
-                  h1 = g[jj].x * xlow - g[jj].y * h1;
-                }
+            for (int jj = PB - 1; jj >= 0; --jj) {
+              const int p = b * PB + jj;
+              if (p >= 1) {
+                const T h0 = x[p >= 1 ? p - 1 : 0];
+                x[p] = g[jj].y * h0 + g[jj].x * h1;
+                h1 = g[jj].x * h0 - g[jj].y * h1;
+              } else if (q > 0) {
+                x[0] = g[jj].y * xlow + g[jj].x * h1;
+                h1 = g[jj].x * xlow - g[jj].y * h1;
               }
             }
           }
-          // where the chain leaves this wave: column c0-1 (wave q-1) takes the
-          // pending value; otherwise, if the window's bottom pair was the lowest
-          // pair applied, its pending value is the final column ilo
-          const int plow = (bbot * PB > (q > 0 ? 0 : 1)) ? bbot * PB : (q > 0 ? 0 : 1);
-          if (q > 0 && pbot == 0) {
-            S.carry[q - 1][lane] = h1;
-          } else if (plow == pbot && bot == ilo) {
+        }
+        // where the chain leaves this wave: column c0-1 (wave q-1) takes the
+        // pending value; otherwise, if the window's bottom pair was the lowest
+        // pair applied, its pending value is the final column ilo
+        const int plow = (bbot * PB > (q > 0 ? 0 : 1)) ? bbot * PB : (q > 0 ? 0 : 1);
+        if (q > 0 && pbot == 0) {
+          S.carry[q - 1][lane] = h1;
+        } else if (plow == pbot && bot == ilo) {
 #pragma unroll
-            for (int j = 0; j < CW; ++j)
-              if (j == pbot - 1) x[j] = h1;
-          }
+          for (int j = 0; j < CW; ++j)
+            if (j == pbot - 1) x[j] = h1;
         }
       }
       S.topv[q][lane] = x[CW - 1];
