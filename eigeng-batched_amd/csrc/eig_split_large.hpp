@@ -708,8 +708,8 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
           h1 = S.carry[q][lane];
         } else {
 #pragma unroll
-          for (int j = 0; j < CW; ++j)
-            if (j == btop * PB + PB - 1) h1 = x[j];
+          for (int bb = 0; bb < CW / PB; ++bb)                  // x[btop PB + PB - 1]
+            if (bb == btop) h1 = x[bb * PB + PB - 1];
         }
         const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
         T2 g[PB];
