@@ -104,3 +104,38 @@ int main() {
     assert e1 == 0 and e2 == 0 and e3 != 0
     assert lw_f == eg.bufsize(16, 64, 64, 64, 4) and lw_d == eg.bufsize(16, 33, 33, 33, 8)
     assert lw_bad == 0 and q == lw_f
+
+
+@pytest.mark.parametrize("sfx", ["FP", "DP"])
+def test_products_validate_arguments_without_a_gpu(sfx):
+    """The product entry points reject bad sizes and null pointers before any
+    HIP call (hipErrorInvalidValue = 1), and L == 0 is a successful no-op."""
+    lib = eg.lib()
+    sym = getattr(lib, "eigen_GPU_batch_symdmt_" + sfx)
+    mp = getattr(lib, "eigen_GPU_batch_matpow_" + sfx)
+    gr = getattr(lib, "eigen_GPU_batch_gram_" + sfx)
+    z, d, c = 4096, 8192, 1 << 20                  # never dereferenced: every call returns first
+    for L, nm, n in ((1, 4, 0), (1, 3, 4), (-1, 4, 4), (1, 513, 513)):
+        assert sym(L, nm, n, z, d, c, None) == 1
+        assert mp(L, nm, n, z, d, -0.5, 1.0, c, None) == 1
+        assert gr(L, nm, n, 8, z, 8, None, 0, 1.0, c, None) == 1
+    assert sym(1, 4, 4, z, d, None, None) == 1      # null output
+    assert gr(1, 4, 4, 8, z, 7, None, 0, 1.0, c, None) == 1     # ldy < p
+    assert gr(1, 4, 4, 8, z, 8, None, -1, 1.0, c, None) == 1    # negative rinv stride
+    assert sym(0, 4, 4, z, d, c, None) == 0
+    assert gr(0, 4, 4, 8, z, 8, None, 0, 1.0, c, None) == 0
+
+
+def test_uneven_workspace_sizes():
+    """The uneven workspace: zero for invalid arguments; beyond order 64 it
+    holds the split route's slots (chunks of 32768 FP32 / 8192 FP64 matrices)."""
+    assert eg.uneven_bufsize(0, 64, 4) == 0 and eg.uneven_bufsize(10, 0, 4) == 0
+    assert eg.uneven_bufsize(10, 513, 4) == 0 and eg.uneven_bufsize(10, 64, 2) == 0
+    small = eg.uneven_bufsize(100000, 64, 4)
+    big = eg.uneven_bufsize(100000, 128, 4)
+    assert small < 64 << 20 < big
+    # slots of order 128: W, rotation log, sweep log, d/e, betas, output slots
+    per = 128 * 128 * 4 + (3 * 128 * 128 // 2 + 16) * 8 + (3 * 128 + 8) * 4 + 2 * 128 * 4 + 128 * 4 + 128 * 4 + 8
+    assert 32768 * per <= big <= 32768 * per + small + (256 * 128 * 128 * 4) + (64 << 10)
+    assert eg.uneven_bufsize(1000, 128, 4) < big                 # chunks never exceed the batch
+    assert eg.uneven_bufsize(100000, 200, 8) > eg.uneven_bufsize(100000, 64, 8)
