@@ -62,11 +62,13 @@ def test_uneven_packed_vs_oracle(dtype, lo, hi, nmax, solver):
     check_against_oracle(mats, res, dtype, solver)
 
 
+@pytest.mark.parametrize("route", ["auto", "split"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_uneven_offsets_gaps_and_bad_orders(dtype):
+def test_uneven_offsets_gaps_and_bad_orders(dtype, route):
     """Explicit offsets in reverse order with NaN gaps; orders 0 and > nmax are
-    flagged -3 and their storage is left untouched."""
-    print(f"[uneven] offsets dtype={np.dtype(dtype).name}", flush=True)
+    flagged -3 and their storage is left untouched (also through the split
+    route of the orders beyond 64)."""
+    print(f"[uneven] offsets dtype={np.dtype(dtype).name} route={route}", flush=True)
     nmax = 70
     ns = list(random_orders(60, 1, nmax, seed=3))
     ns[5] = 0
@@ -89,7 +91,8 @@ def test_uneven_offsets_gaps_and_bad_orders(dtype):
     nv = torch.tensor(ns, dtype=torch.int32, device=DEV)
     ao = torch.tensor(aoff, dtype=torch.int64, device=DEV)
     wo = torch.tensor(woff, dtype=torch.int64, device=DEV)
-    a, w, info = eg.eigen_batch_uneven(a, nv, nmax, w=w, aoff=ao, woff=wo)
+    solver = eg.IMTQL2 | (eg.FLAG_SPLIT if route == "split" else 0)
+    a, w, info = eg.eigen_batch_uneven(a, nv, nmax, w=w, aoff=ao, woff=wo, solver=solver)
     torch.cuda.synchronize()
     info = info.cpu().numpy()
     a_o, w_o = a.cpu().numpy(), w.cpu().numpy()
