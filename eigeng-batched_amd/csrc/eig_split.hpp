@@ -178,46 +178,59 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
       if (st == 0) dS[l][lane] += shift;
     }
   } else {
-    // implicit shift (imtql2_tiled.hpp:235-321)
-    for (int l = 0; l < n && st == 0; ++l) {
-      for (int itr = 0;; ++itr) {
-        if (itr > QL_MAX_SWEEP) { st = l + 1; break; }
-        int m = l;
+    // implicit shift (imtql2_tiled.hpp:235-321) as one flat loop: every pass
+    // runs one sweep of the lane's current eigenvalue l, moving l on past the
+    // converged ones first.  Nested loops over l and its sweeps would make the
+    // lanes of a wave wait for each other at every l (a wave's lanes leave a
+    // loop together), costing the maximum over lanes of the sweeps of each l
+    // instead of the maximum of their sums.  Each lane's sequence of sweeps,
+    // and so its arithmetic, is the nested form's.
+    int l = 0, itr = 0;
+    while (l < n) {
+      int m = l;
+      for (;;) {
+        if (itr > QL_MAX_SWEEP) break;
+        m = l;
         while (m < n - 1 && abs_(eS[m][lane]) > (abs_(dS[m][lane]) + abs_(dS[m + 1][lane])) * tol) ++m;
-        if (m == l) break;
-        if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; break; }
-        const T dl = dS[l][lane], el = eS[l][lane];
-        const T f0 = (dS[l + 1][lane] - dl) / (el + el);
-        const T g0 = hypot1_c(f0);
-        T di1 = dS[m][lane];
-        T r = (di1 - dl) + el / (f0 + sign_(g0, f0));
-        T s = T(1), c = T(1), delta = T(0);
-        T ei = eS[m - 1][lane], di = dS[m - 1][lane];
-        int i;
-        for (i = m - 1; i >= l; --i) {
-          const int ip = (i > 0) ? i - 1 : 0;
-          const T ein = eS[ip][lane], din = dS[ip][lane];
-          const T f = s * ei, gg = r;
-          const T bb = c * ei;
-          if (!givens(f, gg, r, c, s)) break;
-          const T dix = di1 - delta;
-          di1 = di;
-          const T q = (di1 - dix) * s + T(2) * c * bb;
-          const T dx1 = fma_(s, q, dix);
-          delta = dx1 - dix;
-          eS[i + 1][lane] = r;
-          dS[i + 1][lane] = dx1;
-          lg[nrot++] = T2{c, s};
-          r = c * q - bb;
-          ei = ein;
-          di = din;
-        }
-        const int j = (i + 1 > l) ? i + 1 : l;
-        dS[j][lane] -= delta;
-        eS[j][lane] = r;
-        eS[m][lane] = T(0);
-        sw[nsw++] = sw_code(l, m, (i + 1 > l) ? i + 1 : l);
+        if (m != l) break;
+        itr = 0;
+        if (++l >= n) break;
       }
+      if (l >= n) break;                           // every eigenvalue converged
+      if (itr > QL_MAX_SWEEP) { st = l + 1; break; }
+      ++itr;
+      if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; break; }
+      const T dl = dS[l][lane], el = eS[l][lane];
+      const T f0 = (dS[l + 1][lane] - dl) / (el + el);
+      const T g0 = hypot1_c(f0);
+      T di1 = dS[m][lane];
+      T r = (di1 - dl) + el / (f0 + sign_(g0, f0));
+      T s = T(1), c = T(1), delta = T(0);
+      T ei = eS[m - 1][lane], di = dS[m - 1][lane];
+      int i;
+      for (i = m - 1; i >= l; --i) {
+        const int ip = (i > 0) ? i - 1 : 0;
+        const T ein = eS[ip][lane], din = dS[ip][lane];
+        const T f = s * ei, gg = r;
+        const T bb = c * ei;
+        if (!givens(f, gg, r, c, s)) break;
+        const T dix = di1 - delta;
+        di1 = di;
+        const T q = (di1 - dix) * s + T(2) * c * bb;
+        const T dx1 = fma_(s, q, dix);
+        delta = dx1 - dix;
+        eS[i + 1][lane] = r;
+        dS[i + 1][lane] = dx1;
+        lg[nrot++] = T2{c, s};
+        r = c * q - bb;
+        ei = ein;
+        di = din;
+      }
+      const int j = (i + 1 > l) ? i + 1 : l;
+      dS[j][lane] -= delta;
+      eS[j][lane] = r;
+      eS[m][lane] = T(0);
+      sw[nsw++] = sw_code(l, m, (i + 1 > l) ? i + 1 : l);
     }
   }
 
