@@ -130,52 +130,66 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
   int nrot = 0, nsw = 0, st = 0;
 
   if (solver == SOLVER_TQL2) {
-    // explicit shift (tql2_tiled.hpp:40-146)
+    // explicit shift (tql2_tiled.hpp:40-146), as one flat loop of sweeps per
+    // lane like the implicit-shift form below: l is set up (tst1, m; the
+    // converged ones finished) when its predecessor converges, every pass
+    // runs one sweep.  Each lane's arithmetic is the nested form's.
     T shift = T(0), tst1 = T(0);
-    for (int l = 0; l < n && st == 0; ++l) {
-      const T h = (abs_(dS[l][lane]) + abs_(eS[l][lane])) * tol;
-      tst1 = (h > tst1) ? h : tst1;
-      int m = l;
-      while (m < n - 1 && abs_(eS[m][lane]) > tst1) ++m;
-      if (m != l) {
-        int itr;
-        for (itr = 0; itr < QL_MAX_SWEEP; ++itr) {
-          if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; break; }
-          const T dl_old = dS[l][lane], el = eS[l][lane];
-          const T p0 = (dS[l + 1][lane] - dl_old) / (el + el);
-          const T r0 = hypot1_c(p0);
-          const T psr = p0 + sign_(r0, p0);
-          const T dl = el / psr, dl1 = el * psr;
-          const T delta = dl_old - dl;
-          dS[l][lane] = dl;
-          dS[l + 1][lane] = dl1;
-          for (int i = l + 2; i < n; ++i) dS[i][lane] -= delta;
-          shift += delta;
-          T c = T(1), c2 = T(1), c3 = T(1), s = T(0), s2 = T(0);
-          T pp = dS[m][lane];
-          const T el1 = eS[l + 1][lane];
-          for (int i = m - 1; i >= l; --i) {
-            c3 = c2; c2 = c; s2 = s;
-            const T ei = eS[i][lane], di = dS[i][lane];
-            const T gg = c * ei, hh = c * pp;
-            T r, cn, sn;
-            if (!givens(ei, pp, r, cn, sn)) { cn = T(1); sn = T(0); }
-            eS[i + 1][lane] = s * r;
-            s = sn;
-            c = cn;
-            pp = c * di - s * gg;
-            dS[i + 1][lane] = hh + s * (c * gg + s * di);
-            lg[nrot++] = T2{c, s};
-          }
-          const T rr = (-s * s2 * c3 * el1 * el) / dl1;
-          eS[l][lane] = s * rr;
-          dS[l][lane] = c * rr;
-          sw[nsw++] = sw_code(l, m, l);
-          if (abs_(eS[l][lane]) <= tst1) break;
+    int l = 0, m = 0, itr = 0;
+    bool fresh = true;
+    while (l < n) {
+      if (fresh) {
+        for (;;) {
+          const T h = (abs_(dS[l][lane]) + abs_(eS[l][lane])) * tol;
+          tst1 = (h > tst1) ? h : tst1;
+          m = l;
+          while (m < n - 1 && abs_(eS[m][lane]) > tst1) ++m;
+          if (m != l) break;
+          dS[l][lane] += shift;
+          if (++l >= n) break;
         }
-        if (st == 0 && itr >= QL_MAX_SWEEP) st = l + 1;
+        if (l >= n) break;
+        fresh = false;
+        itr = 0;
       }
-      if (st == 0) dS[l][lane] += shift;
+      if (itr >= QL_MAX_SWEEP) { st = l + 1; break; }
+      if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; break; }
+      const T dl_old = dS[l][lane], el = eS[l][lane];
+      const T p0 = (dS[l + 1][lane] - dl_old) / (el + el);
+      const T r0 = hypot1_c(p0);
+      const T psr = p0 + sign_(r0, p0);
+      const T dl = el / psr, dl1 = el * psr;
+      const T delta = dl_old - dl;
+      dS[l][lane] = dl;
+      dS[l + 1][lane] = dl1;
+      for (int i = l + 2; i < n; ++i) dS[i][lane] -= delta;
+      shift += delta;
+      T c = T(1), c2 = T(1), c3 = T(1), s = T(0), s2 = T(0);
+      T pp = dS[m][lane];
+      const T el1 = eS[l + 1][lane];
+      for (int i = m - 1; i >= l; --i) {
+        c3 = c2; c2 = c; s2 = s;
+        const T ei = eS[i][lane], di = dS[i][lane];
+        const T gg = c * ei, hh = c * pp;
+        T r, cn, sn;
+        if (!givens(ei, pp, r, cn, sn)) { cn = T(1); sn = T(0); }
+        eS[i + 1][lane] = s * r;
+        s = sn;
+        c = cn;
+        pp = c * di - s * gg;
+        dS[i + 1][lane] = hh + s * (c * gg + s * di);
+        lg[nrot++] = T2{c, s};
+      }
+      const T rr = (-s * s2 * c3 * el1 * el) / dl1;
+      eS[l][lane] = s * rr;
+      dS[l][lane] = c * rr;
+      sw[nsw++] = sw_code(l, m, l);
+      ++itr;
+      if (abs_(eS[l][lane]) <= tst1) {             // eigenvalue l converged
+        dS[l][lane] += shift;
+        ++l;
+        fresh = true;
+      }
     }
   } else {
     // implicit shift (imtql2_tiled.hpp:235-321) as one flat loop: every pass
