@@ -61,27 +61,37 @@ __device__ __forceinline__ int sw_m(int code) { return (code >> 10) & 1023; }
 __device__ __forceinline__ int sw_ilo(int code) { return (code >> 20) & 1023; }
 
 template <typename T, int NB>
-__global__ __launch_bounds__(64) void eig_tri_kernel(int L, int nm, int n, const T *__restrict__ a,
-                                                     T *__restrict__ qws, T *__restrict__ de, int ld)
+__global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_tri_kernel(int L, int nm, int n,
+                                                                        const T *__restrict__ a,
+                                                                        T *__restrict__ qws,
+                                                                        T *__restrict__ de, int ld)
 {
-  static_assert(GroupShape<NB>::NW == 1, "split path is for n <= 64");
-  constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NW = 1;
+  // NB <= 64: groups of NB lanes; NB = 128: two waves per matrix, row k of
+  // the matrix in lane k % 64 of wave k / 64
+  constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
   __shared__ WaveLds<T, NB> smem[G];
-  const int lane = threadIdx.x & 63;
-  const int g = lane / NL, kl = lane % NL, k = kl;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane / NL, kl = lane % NL, k = kl + NL * wv;
   const long long mat = (long long)blockIdx.x * G + g;
-  if (mat >= L) return;
+  if (mat >= L) return;                              // the whole group (G = 1 when NW > 1)
   WaveLds<T, NB> &S = smem[g];
   const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
   T R[NB], dv[NW], ev[NW];
-  wave_front<T, NB>(A, nm, n, k, kl, 0, S, R, dv, ev);
+  wave_front<T, NB>(A, nm, n, k, kl, wv, S, R, dv, ev);
   T *__restrict__ Q = qws + (size_t)mat * (size_t)n * (size_t)n;
 #pragma unroll
   for (int j = 0; j < NB; ++j)
     if (j < n && k < n) Q[k + (size_t)j * n] = R[j];
-  if (k < n) {
-    de[(size_t)k * ld + mat] = dv[0];
-    de[(size_t)(n + k) * ld + mat] = ev[0];
+  if (wv == 0) {
+    // d and e are lane-distributed (entry kl + NL h in dv[h]) and the same in every wave
+#pragma unroll
+    for (int h = 0; h < NW; ++h) {
+      const int i = kl + NL * h;
+      if (i < n) {
+        de[(size_t)i * ld + mat] = dv[h];
+        de[(size_t)(n + i) * ld + mat] = ev[h];
+      }
+    }
   }
 }
 
@@ -502,6 +512,10 @@ struct alignas(16) ApplyLds {
   alignas(16) int pos[NB];
 };
 
+// NB <= 64: groups of NB lanes, lane k = row k of Q.  NB = 128: one wave per
+// 64-row block of a matrix (NR = 2 workgroups per matrix), each lane a full
+// row of 128 columns; the blocks share nothing, every wave stages the
+// rotations of a sweep (up to NB - 1) itself, NR per lane.
 template <typename T, int NB>
 __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *__restrict__ a,
                                                        const T *__restrict__ qws,
@@ -511,12 +525,12 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
                                                        const int *__restrict__ status, const int *__restrict__ slot)
 {
   typedef typename vec2<T>::type T2;
-  constexpr int G = GroupShape<NB>::G;
-  static_assert(GroupShape<NB>::NW == 1, "split path is for n <= 64");
+  constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NR = NB / NL;
   __shared__ ApplyLds<T, NB> smem[G];
   const int lane = threadIdx.x & 63;
-  const int g = lane / NB, k = lane % NB;
-  const long long mat = (long long)blockIdx.x * G + g;
+  const int g = lane / NL, kl = lane % NL;
+  const long long mat = (NR == 1) ? (long long)blockIdx.x * G + g : (long long)(blockIdx.x / NR);
+  const int k = (NR == 1) ? kl : kl + NL * (int)(blockIdx.x % NR);   // row of Q
   if (mat >= L) return;
   if (status[mat] != 0) return;
   ApplyLds<T, NB> &S = smem[g];
@@ -527,36 +541,48 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
   T R[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) R[j] = (j < n && k < n) ? Q[k + (size_t)j * n] : T(0);
-  S.cs[k] = T2{T(1), T(0)};
+#pragma unroll
+  for (int t = 0; t < NR; ++t) S.cs[kl + NL * t] = T2{T(1), T(0)};
   const int nsw = nsw_in[mat];
-  // software pipeline: the next sweep's header and this lane's log entry are
-  // loaded while the current sweep's chain runs
+  // software pipeline: the next sweep's header and this lane's log entries
+  // (positions kl + NL t of the sweep) are loaded while the current chain runs
   int code = (nsw > 0) ? sw[0] : 0;
   int base = 0;
-  T2 q = T2{T(1), T(0)};
+  T2 q[NR];
   {
     const int m = sw_m(code), ilo = sw_ilo(code);
-    if (nsw > 0 && k < m - ilo) q = lg[k];
+#pragma unroll
+    for (int t = 0; t < NR; ++t) q[t] = (nsw > 0 && kl + NL * t < m - ilo) ? lg[kl + NL * t] : T2{T(1), T(0)};
   }
   for (int s = 0; s < nsw; ++s) {
     const int m = sw_m(code), ilo = sw_ilo(code);
     const int cnt = m - ilo;
-    if (k < cnt) S.cs[m - 1 - k] = q;
+#pragma unroll
+    for (int t = 0; t < NR; ++t)
+      if (kl + NL * t < cnt) S.cs[m - 1 - (kl + NL * t)] = q[t];
     base += cnt;
     // prefetch sweep s+1
     const int code2 = (s + 1 < nsw) ? sw[s + 1] : 0;
     {
       const int m2 = sw_m(code2), ilo2 = sw_ilo(code2);
-      if (s + 1 < nsw && k < m2 - ilo2) q = lg[base + k];
+#pragma unroll
+      for (int t = 0; t < NR; ++t)
+        if (s + 1 < nsw && kl + NL * t < m2 - ilo2) q[t] = lg[base + kl + NL * t];
     }
     wsync();
     apply_sweep_rotations<T, NB>(R, S.cs, ilo, m - 1);
     wsync();
-    if (k >= ilo && k <= m - 1) S.cs[k] = T2{T(1), T(0)};
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+      const int i = kl + NL * t;
+      if (i >= ilo && i <= m - 1) S.cs[i] = T2{T(1), T(0)};
+    }
     code = code2;
     wsync();
   }
-  if (k < n) S.pos[k] = slot[(size_t)mat * n + k];
+#pragma unroll
+  for (int t = 0; t < NR; ++t)
+    if (kl + NL * t < n) S.pos[kl + NL * t] = slot[(size_t)mat * n + kl + NL * t];
   wsync();
   T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
 #pragma unroll

@@ -48,6 +48,9 @@ constexpr int FLAG_QL_LDS = 0x800;
 // the phases hand over through the workspace, so consecutive calls with the
 // same workspace and stream compose the full solve (batches of at most LCHUNK)
 constexpr int FLAG_PH_TRI = 0x1000, FLAG_PH_QL = 0x2000, FLAG_PH_APPLY = 0x4000;
+// FP32 64 < n <= 128: the register split path (two-wave tridiagonalisation,
+// lane-per-matrix QL, 64-row-block apply) instead of the large-order one
+constexpr int FLAG_REG128 = 0x8000;
 constexpr int FLAG_PH_ALL = FLAG_PH_TRI | FLAG_PH_QL | FLAG_PH_APPLY;
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -114,6 +117,19 @@ bool use_split_large(int L, int n, int tbytes, int solver)
   return L >= (use_wave_kernel(n, tbytes) ? SPLIT_MIN_L : SPLIT_LARGE_MIN_L);
 }
 
+// FP32 64 < n <= 128: the register split path is the default from
+// SPLIT_MIN_L matrices on (measured at L = 8192: n = 128 12.3 ms, against
+// 24.2 ms for the large-order split path and 59 ms for the fused kernel);
+// FLAG_SPLIT still selects the large-order split path
+bool use_split128(int L, int n, int tbytes, int solver)
+{
+  const int base = solver & 0xff, flags = solver & ~0xff;
+  if (tbytes != 4 || n <= 64 || n > 128 || (base != SOLVER_IMTQL2 && base != SOLVER_TQL2)) return false;
+  if (flags & (FLAG_FUSED | FLAG_SPLIT)) return false;
+  if (flags & FLAG_REG128) return true;
+  return L >= SPLIT_MIN_L;
+}
+
 int global_grid(int L) { return L < 1024 ? L : 1024; }
 
 // Workspace: the register kernels keep everything in registers and LDS (a
@@ -126,7 +142,8 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
   (void)nm; (void)m;
   solver = tuned_solver(L, n, tbytes, solver);
-  if (use_split(L, n, solver)) return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
+  if (use_split(L, n, solver) || use_split128(L, n, tbytes, solver))
+    return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
   if (n > GL_NMAX) return 0;
   if (use_split_large(L, n, tbytes, solver))
     return WS_HEAD + SplitLargeLayout(L < LCHUNK_LARGE ? L : LCHUNK_LARGE, n, tbytes).total;
@@ -149,7 +166,7 @@ template <typename T, int NB>
 hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
 {
   typedef typename vec2<T>::type T2;
-  constexpr int G = GroupShape<NB>::G;
+  constexpr int G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
   const int lc = L < LCHUNK ? L : LCHUNK;
   const SplitLayout lay(lc, n, (int)sizeof(T));
   char *base = (char *)wk + WS_HEAD;
@@ -167,7 +184,7 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
     const unsigned gb = (unsigned)((cur + G - 1) / G), lb = (unsigned)((cur + 63) / 64);
     const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
     if (ph & FLAG_PH_TRI)
-      hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, de, lc);
+      hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64 * NW), 0, s, cur, nm, n, ac, q, de, lc);
     if (ph & FLAG_PH_QL) {
       // registers hold d and e (2 NB values per lane): FP32 up to NB = 64, FP64 up to 32
       const bool reg = (solver & 0xff) == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS) && NB * sizeof(T) <= 256;
@@ -179,9 +196,10 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
                            sw, scap, nsw, st, slot, wc, ic, solver & 0xff);
     }
     if (ph & FLAG_PH_APPLY) {
-      hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
+      // NB = 128: one workgroup per 64-row block (NW per matrix)
+      hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb * NW), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
                          sw, scap, nsw, st, slot);
-      hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3(gb), dim3(64), 0, s, cur, nm, n, ac, wc, ic,
+      hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3(gb), dim3(64 * NW), 0, s, cur, nm, n, ac, wc, ic,
                          solver & 0xff, (const int *)st);
     }
   }
@@ -381,8 +399,12 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   const int base = solver & 0xff;
   if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2 && base != SOLVER_JACOBI) return hipErrorNotSupported;
   if (wk == nullptr) return hipErrorInvalidValue;
-  if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver) && !use_split_large(L, n, (int)sizeof(T), solver))
+  if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver) && !use_split_large(L, n, (int)sizeof(T), solver) &&
+      !use_split128(L, n, (int)sizeof(T), solver))
     return hipErrorInvalidValue;
+  if constexpr (sizeof(T) == 4) {
+    if (use_split128(L, n, (int)sizeof(T), solver)) return launch_split<T, 128>(L, nm, n, a, w, info, solver, wk, s);
+  }
   if (use_split(L, n, solver)) {
     if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, wk, s);
     if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, wk, s);
@@ -423,6 +445,7 @@ std::vector<int> tune_candidates(int L, int n, int tb, int solver)
       c.push_back(FLAG_SPLIT);
       return c;
     }
+    c.push_back(FLAG_REG128);
     c.push_back(FLAG_SPLIT | (1 << TRI_CFG_SHIFT));
     c.push_back(FLAG_SPLIT | (2 << TRI_CFG_SHIFT));
     c.push_back(FLAG_SPLIT | TRI_FULL_BIT | (2 << TRI_CFG_SHIFT));

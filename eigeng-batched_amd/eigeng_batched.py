@@ -24,8 +24,10 @@ IMTQL2, TQL2, JACOBI = 0, 1, 2
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
 FLAG_QL_LDS = 0x800
 FLAG_TRI_FULL = 0x4000000
+FLAG_REG128 = 0x8000
 PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG,
-         "split-lds": FLAG_SPLIT | FLAG_QL_LDS, "split-full": FLAG_SPLIT | FLAG_TRI_FULL}
+         "split-lds": FLAG_SPLIT | FLAG_QL_LDS, "split-full": FLAG_SPLIT | FLAG_TRI_FULL,
+         "split-reg128": FLAG_REG128, "split-reg128-fallback": FLAG_REG128 | FLAG_TINYLOG}
 # split-path phases (a call with a subset runs only those kernels; the phases
 # hand over through the workspace, so tri -> ql -> apply calls on one stream
 # and one workspace compose the full solve, for batches of at most 16384)

@@ -152,17 +152,21 @@ def test_large_n_vs_oracle(n, dtype):
 LARGE_SPLIT_SIZES = [65, 96, 129, 160, 200, 255, 256, 257, 300, 384, 512]
 
 
-@pytest.mark.parametrize("path", ["split", "split-fallback", "split-full"])
+@pytest.mark.parametrize("path", ["split", "split-fallback", "split-full", "split-reg128", "split-reg128-fallback"])
 @pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", LARGE_SPLIT_SIZES)
 def test_large_split_vs_oracle(n, dtype, solver, path):
-    """Orders beyond the register kernels through the split path (workgroup
-    tridiagonalisation, lane-per-matrix QL with rotation log, row-block apply),
-    through its overflow fallback (log capacity forced to 1) and, for n <= 256,
-    with the full-storage tridiagonalisation instead of the lower-triangle one."""
+    """Orders beyond 64 through the split path (workgroup tridiagonalisation,
+    lane-per-matrix QL with rotation log, row-block apply), through its
+    overflow fallback (log capacity forced to 1), for n <= 256 with the
+    full-storage tridiagonalisation instead of the lower-triangle one, and, for
+    FP32 n <= 128, through the register split path (two-wave
+    tridiagonalisation, 64-row-block apply) and its fallback."""
     if path == "split-full" and n > 256:
         pytest.skip("the full-storage form is the only one beyond n = 256")
+    if path.startswith("split-reg128") and (dtype != np.float32 or n > 128):
+        pytest.skip("the register split path serves FP32 64 < n <= 128")
     print(f"[parity] large {path} solver={solver} n={n} dtype={np.dtype(dtype).name}", flush=True)
     nm = n + (n & 1)
     L = 7 if n <= 256 else 3
@@ -176,12 +180,14 @@ def test_large_split_vs_oracle(n, dtype, solver, path):
     check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
 
 
-@pytest.mark.parametrize("path", ["split", "split-full", "fused"])
+@pytest.mark.parametrize("path", ["split", "split-full", "fused", "split-reg128"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("n", [65, 129, 256, 300])
+@pytest.mark.parametrize("n", [65, 128, 129, 256, 300])
 def test_large_upper_triangle_only(n, dtype, path):
     """Only the upper triangle (with the diagonal) is read: NaN in the strictly
     lower triangle changes nothing, bit for bit."""
+    if path == "split-reg128" and (dtype != np.float32 or n > 128):
+        pytest.skip("the register split path serves FP32 64 < n <= 128")
     print(f"[parity] upper-only {path} n={n} dtype={np.dtype(dtype).name}", flush=True)
     L = 5
     a0 = wl.sym_rand(L, n, n, dtype=dtype, seed=7100 + n)

@@ -101,7 +101,7 @@ def run_mine_path(a_np, path):
 @pytest.mark.parametrize("dtype,n,path", [(np.float32, 64, "split"), (np.float32, 48, "split"),
                                           (np.float64, 32, "split"), (np.float64, 64, "split"),
                                           (np.float64, 200, "split"), (np.float32, 100, "split"),
-                                          (np.float32, 256, "split")])
+                                          (np.float32, 100, "split-reg128"), (np.float32, 256, "split")])
 def test_split_paths_match_reference(reflib, n, dtype, path):
     """The split paths (register QL, rotation logs, large-order pipeline) against
     the reference library on the same LETKF-like inputs."""
