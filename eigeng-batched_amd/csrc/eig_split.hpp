@@ -60,25 +60,44 @@ __device__ __forceinline__ int sw_code(int l, int m, int ilo) { return l | (m <<
 __device__ __forceinline__ int sw_m(int code) { return (code >> 10) & 1023; }
 __device__ __forceinline__ int sw_ilo(int code) { return (code >> 20) & 1023; }
 
-template <typename T, int NB>
+// LIST (NB = 128 only): the uneven-batch form (u), one workgroup per slot of
+// the chunk, its own order n from nv, packed input at a + aoff, Q slot nmx^2
+// (leading dimension n), d / e at rows i / nmx + i of de.
+template <typename T, int NB, bool LIST = false>
 __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_tri_kernel(int L, int nm, int n,
                                                                         const T *__restrict__ a,
                                                                         T *__restrict__ qws,
-                                                                        T *__restrict__ de, int ld)
+                                                                        T *__restrict__ de, int ld,
+                                                                        UvSplitArgs u = UvSplitArgs{})
 {
   // NB <= 64: groups of NB lanes; NB = 128: two waves per matrix, row k of
   // the matrix in lane k % 64 of wave k / 64
   constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
+  static_assert(!LIST || G == 1, "the list form takes one matrix per workgroup");
   __shared__ WaveLds<T, NB> smem[G];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane / NL, kl = lane % NL, k = kl + NL * wv;
-  const long long mat = (long long)blockIdx.x * G + g;
-  if (mat >= L) return;                              // the whole group (G = 1 when NW > 1)
+  const long long mat = (long long)blockIdx.x * G + g;  // the slot (LIST) or the matrix
+  const T *__restrict__ A;
+  T *__restrict__ Q;
+  int eoff = n;
+  if constexpr (LIST) {
+    int mi;
+    if (!uv_split_entry(u, mat, mi)) return;         // the whole workgroup
+    n = __builtin_amdgcn_readfirstlane(u.nv[mi]);
+    nm = n;
+    eoff = u.nmx;
+    A = a + u.aoff[mi];
+    Q = qws + (size_t)mat * u.nmx * u.nmx;
+  } else {
+    if (mat >= L) return;                            // the whole group (G = 1 when NW > 1)
+    eoff = n;
+    A = a + (size_t)mat * (size_t)nm * (size_t)n;
+    Q = qws + (size_t)mat * (size_t)n * (size_t)n;
+  }
   WaveLds<T, NB> &S = smem[g];
-  const T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
   T R[NB], dv[NW], ev[NW];
   wave_front<T, NB>(A, nm, n, k, kl, wv, S, R, dv, ev);
-  T *__restrict__ Q = qws + (size_t)mat * (size_t)n * (size_t)n;
 #pragma unroll
   for (int j = 0; j < NB; ++j)
     if (j < n && k < n) Q[k + (size_t)j * n] = R[j];
@@ -89,7 +108,7 @@ __global__ __launch_bounds__(64 * GroupShape<NB>::NW) void eig_tri_kernel(int L,
       const int i = kl + NL * h;
       if (i < n) {
         de[(size_t)i * ld + mat] = dv[h];
-        de[(size_t)(n + i) * ld + mat] = ev[h];
+        de[(size_t)(eoff + i) * ld + mat] = ev[h];
       }
     }
   }
@@ -516,27 +535,45 @@ struct alignas(16) ApplyLds {
 // 64-row block of a matrix (NR = 2 workgroups per matrix), each lane a full
 // row of 128 columns; the blocks share nothing, every wave stages the
 // rotations of a sweep (up to NB - 1) itself, NR per lane.
-template <typename T, int NB>
+// LIST (NB = 128 only): the uneven-batch form (u), slot = blockIdx / NR.
+template <typename T, int NB, bool LIST = false>
 __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *__restrict__ a,
                                                        const T *__restrict__ qws,
                                                        const typename vec2<T>::type *__restrict__ rotlog,
                                                        int rot_cap, const int *__restrict__ swlog, int sw_cap,
                                                        const int *__restrict__ nsw_in,
-                                                       const int *__restrict__ status, const int *__restrict__ slot)
+                                                       const int *__restrict__ status, const int *__restrict__ slot,
+                                                       UvSplitArgs u = UvSplitArgs{})
 {
   typedef typename vec2<T>::type T2;
   constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NR = NB / NL;
+  static_assert(!LIST || NR > 1, "the list form takes one matrix per NR workgroups");
   __shared__ ApplyLds<T, NB> smem[G];
   const int lane = threadIdx.x & 63;
   const int g = lane / NL, kl = lane % NL;
   const long long mat = (NR == 1) ? (long long)blockIdx.x * G + g : (long long)(blockIdx.x / NR);
   const int k = (NR == 1) ? kl : kl + NL * (int)(blockIdx.x % NR);   // row of Q
-  if (mat >= L) return;
+  T *__restrict__ A;
+  const T *__restrict__ Q;
+  size_t sstride = n;
+  if constexpr (LIST) {
+    int mi;
+    if (!uv_split_entry(u, mat, mi)) return;
+    n = __builtin_amdgcn_readfirstlane(u.nv[mi]);
+    nm = n;
+    if (k - kl >= n) return;                         // a row block beyond the order
+    A = a + u.aoff[mi];
+    Q = qws + (size_t)mat * u.nmx * u.nmx;
+    sstride = u.nmx;
+  } else {
+    if (mat >= L) return;
+    A = a + (size_t)mat * (size_t)nm * (size_t)n;
+    Q = qws + (size_t)mat * (size_t)n * (size_t)n;
+  }
   if (status[mat] != 0) return;
   ApplyLds<T, NB> &S = smem[g];
   const T2 *__restrict__ lg = rotlog + (size_t)mat * rot_cap;
   const int *__restrict__ sw = swlog + (size_t)mat * sw_cap;
-  const T *__restrict__ Q = qws + (size_t)mat * (size_t)n * (size_t)n;
 
   T R[NB];
 #pragma unroll
@@ -582,9 +619,8 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
   }
 #pragma unroll
   for (int t = 0; t < NR; ++t)
-    if (kl + NL * t < n) S.pos[kl + NL * t] = slot[(size_t)mat * n + kl + NL * t];
+    if (kl + NL * t < n) S.pos[kl + NL * t] = slot[(size_t)mat * sstride + kl + NL * t];
   wsync();
-  T *__restrict__ A = a + (size_t)mat * (size_t)nm * (size_t)n;
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (j < n) {

@@ -644,21 +644,35 @@ void run_uneven_split(int L, int nmx, int *cc, const int *list, const int *nv, c
   const int sv = solver & 0xff;
   const int min_count = (solver & FLAG_SPLIT) ? 1 : (tb == 4 ? UV_SPLIT_MIN_F32 : UV_SPLIT_MIN_F64);
   constexpr int QMAX = sizeof(T) == 4 ? 128 : 256, NLANE = sizeof(T) == 4 ? 64 : 32;
-  const int nrb = (nmx + 63) >> 6;
   for (long long c0 = 0; c0 < L; c0 += cap) {
     const UvSplitArgs u{cc, list, nv, ao, wo, (int)c0, cap, min_count, nmx};
-    hipLaunchKernelGGL((eig_tri_large_list_kernel<T, 256, TriNbp<T>::value, 256, 1, true>), dim3(cap < 1024 ? cap : 1024),
-                       dim3(256), 0, s, u, (const T *)a, Wm, de, bw);
-    hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0, s,
-                       cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
-    if (nmx <= 128)
-      hipLaunchKernelGGL((eig_apply_large_kernel<T, 4, 32, true>), dim3((unsigned)cap * nrb), dim3(256), 0, s, cap,
-                         0, 0, a, (const T *)Wm, (const T *)bw, (const T2 *)rot, rcap, (const int *)sw, scap,
-                         (const int *)nsw, (const int *)st, (const int *)slot, u);
-    else
-      hipLaunchKernelGGL((eig_apply_large_kernel<T, 8, 32, true>), dim3((unsigned)cap * nrb), dim3(512), 0, s, cap,
-                         0, 0, a, (const T *)Wm, (const T *)bw, (const T2 *)rot, rcap, (const int *)sw, scap,
-                         (const int *)nsw, (const int *)st, (const int *)slot, u);
+    if constexpr (sizeof(T) == 4) {
+      // FP32 orders 65..128: the register split kernels (two waves per
+      // matrix in the tridiagonalisation, one workgroup per 64-row block in
+      // the replay), Q in the Wm slots
+      hipLaunchKernelGGL((eig_tri_kernel<T, 128, true>), dim3(cap), dim3(128), 0, s, cap, 0, 0, (const T *)a, Wm,
+                         de, cap, u);
+      hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0,
+                         s, cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+      hipLaunchKernelGGL((eig_apply_kernel<T, 128, true>), dim3((unsigned)cap * 2), dim3(64), 0, s, cap, 0, 0, a,
+                         (const T *)Wm, (const T2 *)rot, rcap, (const int *)sw, scap, (const int *)nsw,
+                         (const int *)st, (const int *)slot, u);
+    } else {
+      // FP64 orders 65..256: the large-order split kernels
+      const int nrb = (nmx + 63) >> 6;
+      hipLaunchKernelGGL((eig_tri_large_list_kernel<T, 256, TriNbp<T>::value, 256, 1, true>),
+                         dim3(cap < 1024 ? cap : 1024), dim3(256), 0, s, u, (const T *)a, Wm, de, bw);
+      hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0,
+                         s, cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+      if (nmx <= 128)
+        hipLaunchKernelGGL((eig_apply_large_kernel<T, 4, 32, true>), dim3((unsigned)cap * nrb), dim3(256), 0, s, cap,
+                           0, 0, a, (const T *)Wm, (const T *)bw, (const T2 *)rot, rcap, (const int *)sw, scap,
+                           (const int *)nsw, (const int *)st, (const int *)slot, u);
+      else
+        hipLaunchKernelGGL((eig_apply_large_kernel<T, 8, 32, true>), dim3((unsigned)cap * nrb), dim3(512), 0, s, cap,
+                           0, 0, a, (const T *)Wm, (const T *)bw, (const T2 *)rot, rcap, (const int *)sw, scap,
+                           (const int *)nsw, (const int *)st, (const int *)slot, u);
+    }
     hipLaunchKernelGGL((eig_global_fallback_list_kernel<T>), dim3(uv_fallback_grid(cap)), dim3(GL_THREADS), 0, s,
                        u, (const int *)st, a, w, info, sv, zws);
   }

@@ -66,6 +66,10 @@ enum eigen_GPU_batch_solver {
  *   0x1000 / 0x2000 / 0x4000  run only the split path's tridiagonalisation /
  *          QL / apply phase (L <= 16384): successive calls on one stream with
  *          one workspace compose the full solve, so each phase can be timed;
+ *   0x8000 FP32 64 < n <= 128: the register split path (two-wave
+ *          tridiagonalisation, lane-per-matrix QL, 64-row-block apply), the
+ *          default there from 2048 matrices; 0x200 selects the large-order
+ *          split path instead;
  *   bits 16-19 / 20-23  workgroup size / resident workgroup count of the
  *          workgroup kernels; bits 24-25 tridiagonalisation shape of the
  *          n > 64 split path; 0x4000000 (n <= 256) its full-storage form
@@ -77,6 +81,7 @@ enum eigen_GPU_batch_solver {
 #define EIGEN_GPU_BATCH_PHASE_TRI 0x1000
 #define EIGEN_GPU_BATCH_PHASE_QL 0x2000
 #define EIGEN_GPU_BATCH_PHASE_APPLY 0x4000
+#define EIGEN_GPU_BATCH_REG128 0x8000
 #define EIGEN_GPU_BATCH_TRI_FULL 0x4000000
 
 /* Workspace size in bytes for (L, nm, n, m, solver); 0 for invalid arguments. */
