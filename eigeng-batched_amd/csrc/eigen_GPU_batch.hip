@@ -555,9 +555,12 @@ int uv_split_nmx(int nmax, int tb)
   const int hi = tb == 4 ? 128 : 256;
   return nmax > 64 ? (nmax < hi ? nmax : hi) : 0;     // 0: no split route
 }
+// FP32: 65536 slots (17 GB at order 128), so that a class of up to 65536
+// matrices pays the latency-bound QL recurrence once (two chunks of 32768 took
+// 6.1 + 2.9 ms of it for 53000 matrices of orders 65..128)
 int uv_split_cap(int L, int tb)
 {
-  const int ch = tb == 4 ? 32768 : 8192;
+  const int ch = tb == 4 ? 65536 : 8192;
   return L < ch ? L : ch;
 }
 int uv_fallback_grid(int cap) { return cap < 256 ? cap : 256; }

@@ -128,7 +128,7 @@ def test_products_validate_arguments_without_a_gpu(sfx):
 
 def test_uneven_workspace_sizes():
     """The uneven workspace: zero for invalid arguments; beyond order 64 it
-    holds the split route's slots (chunks of 32768 FP32 / 8192 FP64 matrices)."""
+    holds the split route's slots (chunks of 65536 FP32 / 8192 FP64 matrices)."""
     assert eg.uneven_bufsize(0, 64, 4) == 0 and eg.uneven_bufsize(10, 0, 4) == 0
     assert eg.uneven_bufsize(10, 513, 4) == 0 and eg.uneven_bufsize(10, 64, 2) == 0
     small = eg.uneven_bufsize(100000, 64, 4)
@@ -136,6 +136,6 @@ def test_uneven_workspace_sizes():
     assert small < 64 << 20 < big
     # slots of order 128: W, rotation log, sweep log, d/e, betas, output slots
     per = 128 * 128 * 4 + (3 * 128 * 128 // 2 + 16) * 8 + (3 * 128 + 8) * 4 + 2 * 128 * 4 + 128 * 4 + 128 * 4 + 8
-    assert 32768 * per <= big <= 32768 * per + small + (256 * 128 * 128 * 4) + (64 << 10)
+    assert 65536 * per <= big <= 65536 * per + small + (256 * 128 * 128 * 4) + (64 << 10)
     assert eg.uneven_bufsize(1000, 128, 4) < big                 # chunks never exceed the batch
     assert eg.uneven_bufsize(100000, 200, 8) > eg.uneven_bufsize(100000, 64, 8)
