@@ -60,7 +60,7 @@ __device__ __forceinline__ int sw_code(int l, int m, int ilo) { return l | (m <<
 __device__ __forceinline__ int sw_m(int code) { return (code >> 10) & 1023; }
 __device__ __forceinline__ int sw_ilo(int code) { return (code >> 20) & 1023; }
 
-// LIST (NB = 128 only): the uneven-batch form (u), one workgroup per slot of
+// LIST (NB = 64 and 128): the uneven-batch form (u), one workgroup per slot of
 // the chunk, its own order n from nv, packed input at a + aoff, Q slot nmx^2
 // (leading dimension n), d / e at rows i / nmx + i of de.
 template <typename T, int NB, bool LIST = false>
@@ -535,7 +535,7 @@ struct alignas(16) ApplyLds {
 // 64-row block of a matrix (NR = 2 workgroups per matrix), each lane a full
 // row of 128 columns; the blocks share nothing, every wave stages the
 // rotations of a sweep (up to NB - 1) itself, NR per lane.
-// LIST (NB = 128 only): the uneven-batch form (u), slot = blockIdx / NR.
+// LIST (NB = 64 and 128): the uneven-batch form (u), slot = blockIdx / NR.
 template <typename T, int NB, bool LIST = false>
 __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *__restrict__ a,
                                                        const T *__restrict__ qws,
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
 {
   typedef typename vec2<T>::type T2;
   constexpr int NL = GroupShape<NB>::NL, G = GroupShape<NB>::G, NR = NB / NL;
-  static_assert(!LIST || NR > 1, "the list form takes one matrix per NR workgroups");
+  static_assert(!LIST || G == 1, "the list form takes one matrix per NR workgroups");
   __shared__ ApplyLds<T, NB> smem[G];
   const int lane = threadIdx.x & 63;
   const int g = lane / NL, kl = lane % NL;

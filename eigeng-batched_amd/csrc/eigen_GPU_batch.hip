@@ -681,6 +681,40 @@ void run_uneven_split(int L, int nmx, int *cc, const int *list, const int *nv, c
   }
 }
 
+// The class (32, 64] on the n <= 64 split kernels in list form (one wave per
+// matrix in the tridiagonalisation and the apply, the LDS QL kernel with an
+// order per lane), in the slots of the split route's region laid out for
+// order 64 (they fit in the region of any larger order)
+template <typename T>
+void run_uneven_split64(int L, int *cc, const int *list, const int *nv, const long long *ao, const long long *wo,
+                        T *a, T *w, int *info, int solver, char *base, T *zws, hipStream_t s)
+{
+  typedef typename vec2<T>::type T2;
+  const int tb = (int)sizeof(T), nmx = 64;
+  const int cap = uv_split_cap(L, tb);
+  const UvSplitLayout lay(cap, nmx, tb);
+  T *Wm = (T *)(base + lay.wm), *de = (T *)(base + lay.de);
+  T2 *rot = (T2 *)(base + lay.rot);
+  int *sw = (int *)(base + lay.sw), *nsw = (int *)(base + lay.nsw);
+  int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
+  const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(nmx), scap = sw_cap_of(nmx);
+  const int sv = solver & 0xff;
+  const int min_count = (solver & FLAG_SPLIT) ? 1 : (tb == 4 ? UV_SPLIT_MIN_F32 : UV_SPLIT_MIN_F64);
+  constexpr int NLANE = sizeof(T) == 4 ? 64 : 32;
+  for (long long c0 = 0; c0 < L; c0 += cap) {
+    const UvSplitArgs u{cc, list, nv, ao, wo, (int)c0, cap, min_count, nmx};
+    hipLaunchKernelGGL((eig_tri_kernel<T, 64, true>), dim3(cap), dim3(64), 0, s, cap, 0, 0, (const T *)a, Wm, de,
+                       cap, u);
+    hipLaunchKernelGGL((eig_ql_lane_kernel<T, 64, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0, s,
+                       cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+    hipLaunchKernelGGL((eig_apply_kernel<T, 64, true>), dim3(cap), dim3(64), 0, s, cap, 0, 0, a, (const T *)Wm,
+                       (const T2 *)rot, rcap, (const int *)sw, scap, (const int *)nsw, (const int *)st,
+                       (const int *)slot, u);
+    hipLaunchKernelGGL((eig_global_fallback_list_kernel<T>), dim3(uv_fallback_grid(cap)), dim3(GL_THREADS), 0, s,
+                       u, (const int *)st, a, w, info, sv, zws);
+  }
+}
+
 template <typename T>
 hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, const long long *woff, T *a, T *w,
                       void *wk, hipStream_t s, int solver, int *info)
@@ -704,7 +738,9 @@ hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, con
     const int lo = c == 0 ? 0 : cls.hi[c - 1];
     if (lo >= nmax) break;
     int *cc = ctl + 4 * c;
-    const bool split = nmx && lo == 64 && !(solver & FLAG_FUSED);
+    // split route: the class beyond order 64, and the class (32, 64] when the
+    // workspace holds the route's region (nmax > 64)
+    const bool split = nmx && (lo == 64 || lo == 32) && !(solver & FLAG_FUSED);
     const int min_count = (solver & FLAG_SPLIT) ? 1 : (tb == 4 ? UV_SPLIT_MIN_F32 : UV_SPLIT_MIN_F64);
     if (split) hipLaunchKernelGGL(uv_claim_kernel, dim3(1), dim3(64), 0, s, cc, min_count);
     switch (cls.hi[c]) {
@@ -717,7 +753,10 @@ hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, con
       hipLaunchKernelGGL((eig_global_list_kernel<T>), dim3(uv_global_grid(L)), dim3(GL_THREADS), 0, s, cc,
                          list, nv, ao, wo, a, w, info, base, zws, (long long)nmax * nmax);
     }
-    if (split) run_uneven_split<T>(L, nmx, cc, list, nv, ao, wo, a, w, info, solver, ws + lay.split, zws, s);
+    if (split && lo == 64)
+      run_uneven_split<T>(L, nmx, cc, list, nv, ao, wo, a, w, info, solver, ws + lay.split, zws, s);
+    else if (split)
+      run_uneven_split64<T>(L, cc, list, nv, ao, wo, a, w, info, solver, ws + lay.split, zws, s);
   }
   return hipGetLastError();
 }
