@@ -148,7 +148,7 @@ def tuned_line(a0, plain_ms, steps, warmup):
             "nonconverged": bad, "accuracy_all": acc}
 
 
-def time_phases(a0, steps, warmup):
+def time_phases(a0, steps, warmup, path="split"):
     """Per-phase HIP-event times (ms) of the split paths: the phases are issued
     as separate library calls on one stream with events between them (n <= 64:
     tri = eig_tri_kernel, ql = eig_ql_reg_kernel, apply = eig_apply_kernel plus
@@ -161,7 +161,7 @@ def time_phases(a0, steps, warmup):
     L, n, _ = a0.shape
     a = torch.empty_like(a0)
     w = torch.empty((L, n), dtype=a0.dtype, device="cuda")
-    ws = torch.empty(max(eg.bufsize(L, n, n, n, a.element_size(), eg.FLAG_SPLIT), 1), dtype=torch.uint8,
+    ws = torch.empty(max(eg.bufsize(L, n, n, n, a.element_size(), eg.PATHS[path]), 1), dtype=torch.uint8,
                      device="cuda")
     info = torch.empty(L, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
@@ -172,7 +172,7 @@ def time_phases(a0, steps, warmup):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         ev[0].record(stream)
         for j, p in enumerate(names):
-            eg.eigen_batch(a, w=w, info=info, workspace=ws, stream=stream, path="split", phases=[p])
+            eg.eigen_batch(a, w=w, info=info, workspace=ws, stream=stream, path=path, phases=[p])
             ev[j + 1].record(stream)
         torch.cuda.synchronize()
         if i >= warmup:
@@ -495,6 +495,38 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
                              "traffic": _traffic("traffic_eig_split_n256_fp64.json")}}
 
 
+def bench_n128(steps, warmup):
+    """L = 8192, n = 128, FP32 line (the register split path, the default for
+    FP32 64 < n <= 128 from 2048 matrices): whole-call time, per-phase times,
+    roofline, accuracy of every matrix."""
+    import torch
+    L2, n2 = 8192, 128
+    q0 = make_qdq_gpu(L2, n2, seed=0xE16E0005, dtype=torch.float32)
+    ms2, _, a2, w2, info2 = time_solver(q0, 0, steps, warmup)
+    acc2 = accuracy_all(q0, a2, w2)
+    bad = int((info2 != 0).sum().item())
+    del a2, w2
+    torch.cuda.empty_cache()
+    ph2, a2, w2, _ = time_phases(q0, 2, 1, path="split-reg128")
+    del q0, a2, w2
+    torch.cuda.empty_cache()
+    kms2 = sum(ph2.values())
+    f2 = (10.0 / 3.0) * n2 ** 3
+    b2 = 4 * (n2 * (n2 + 1) / 2 + n2 * n2 + n2)
+    return {"workload": "L=8192 n=128 FP32, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
+            "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
+            "accuracy_all": acc2,
+            "roofline": {"bound": "mfma", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
+                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                         "kernel": "eig_tri_kernel<float,128> + eig_ql_lane_kernel + eig_apply_kernel<float,128> "
+                                   "(+ overflow-fallback launch)",
+                         "kernel_ms": kms2, "phase_ms": ph2,
+                         "note": "(10/3)n^3 flops per matrix against the FP32 vector peak"},
+            "roofline_hbm": {"achieved": L2 * b2 / (kms2 * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "bytes_per_matrix": b2}}
+
+
 def cpu_baseline(a_host, min_seconds=10.0, max_reps=20):
     import numpy as np
     import oracle
@@ -554,6 +586,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch secondary line")
     ap.add_argument("--no-letkf48", action="store_true", help="skip the L=65536 n=48 line")
+    ap.add_argument("--no-n128", action="store_true", help="skip the n=128 FP32 line")
+    ap.add_argument("--n128-only", action="store_true", help="print only the n=128 FP32 line")
     ap.add_argument("--no-letkf-step", action="store_true", help="skip the LETKF analysis-step line")
     ap.add_argument("--no-vendor", action="store_true", help="skip the hipSOLVER syevjBatched comparison")
     ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned lines")
@@ -579,6 +613,9 @@ def main():
         return
     if args.letkf_step_only:
         print(json.dumps({"letkf_step": bench_letkf_step(args.steps, args.warmup)}))
+        return
+    if args.n128_only:
+        print(json.dumps({"n128": bench_n128(args.steps, args.warmup)}))
         return
     dt = torch.float32 if args.dtype == "fp32" else torch.float64
     L, n = args.L, args.n
@@ -652,6 +689,8 @@ def main():
         out["vendor_baseline"] = vendor_baseline(a0)
     if not args.no_n256:
         out["n256"] = bench_n256(3, 1, vendor=not args.no_vendor, tuned=not args.no_tuned)
+    if not args.no_n128:
+        out["n128"] = bench_n128(5, 1)
     if not args.no_letkf48:
         out["letkf48"] = bench_letkf48(5, 1)
     if not args.no_letkf_step:
