@@ -220,6 +220,34 @@ def test_uneven_split_route_by_count():
         assert np.max(np.abs(x - y)) <= 64 * np.finfo(np.float32).eps * max(np.abs(y).max(), 1e-30)
 
 
+@pytest.mark.parametrize("dtype,count", [(np.float32, 4300), (np.float64, 1100)])
+def test_uneven_split_route_by_count_order64(dtype, count):
+    """Without forcing, with nmax > 64: a class (32, 64] of at least 4096 (FP32) /
+    1024 (FP64) matrices takes the n <= 64 split kernels in list form, while the
+    small class beyond 64 keeps the fused kernel; a sample matches the oracle
+    and every matrix's eigenvalues match the fused kernels'."""
+    print(f"[uneven] by count, class (32,64] dtype={np.dtype(dtype).name} count={count}", flush=True)
+    ns = np.concatenate([random_orders(count, 33, 64, seed=41), random_orders(60, 65, 100, seed=42)])
+    ns = np.random.default_rng(43).permutation(ns)
+    mats = make_mats(ns, dtype, seed=44)
+    a0, nv = eg.pack_uneven([torch.from_numpy(m) for m in mats], DEV)
+    a, w, info = eg.eigen_batch_uneven(a0.clone(), nv, 100)
+    torch.cuda.synchronize()
+    assert int(info.abs().max().item()) == 0
+    res = eg.unpack_uneven(a, w, [int(n) for n in ns])
+    pick = np.random.default_rng(45).choice(len(ns), size=120, replace=False)
+    check_against_oracle([mats[k] for k in pick], [res[k] for k in pick], dtype)
+    a2, w2, info2 = eg.eigen_batch_uneven(a0.clone(), nv, 100, solver=eg.IMTQL2 | eg.FLAG_FUSED)
+    torch.cuda.synchronize()
+    assert int(info2.abs().max().item()) == 0
+    wa, wb = w.cpu().numpy(), w2.cpu().numpy()
+    eps = np.finfo(dtype).eps
+    off = np.concatenate([[0], np.cumsum(ns)])
+    for k in range(len(ns)):
+        x, y = wa[off[k]:off[k + 1]], wb[off[k]:off[k + 1]]
+        assert np.max(np.abs(x - y)) <= 64 * eps * max(np.abs(y).max(), 1e-30)
+
+
 def test_uneven_invalid_args():
     a = torch.zeros(16, device=DEV)
     nv = torch.full((1,), 4, dtype=torch.int32, device=DEV)
