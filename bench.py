@@ -524,7 +524,8 @@ def bench_n128(steps, warmup):
                          "kernel_ms": kms2, "phase_ms": ph2,
                          "note": "(10/3)n^3 flops per matrix against the FP32 vector peak"},
             "roofline_hbm": {"achieved": L2 * b2 / (kms2 * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
-                             "unit": "GB/s", "bytes_per_matrix": b2}}
+                             "unit": "GB/s", "bytes_per_matrix": b2,
+                             "traffic": _traffic("traffic_eig_split128_n128_fp32.json")}}
 
 
 def cpu_baseline(a_host, min_seconds=10.0, max_reps=20):
