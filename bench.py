@@ -293,6 +293,45 @@ def bench_uneven(count, nlo, nhi, steps, warmup, seed=0xE16E0003):
                          "max_rel_residual_F": worst_rel, "max_orth": worst_orth}}
 
 
+def bench_n32(steps, warmup, tuned=True):
+    """L = 1024, n = nm = m = 32, FP64, A = (B + B^T)/2 with B ~ U[-1,1] (seeded,
+    generated on the GPU): time, HBM and VALU roofs of the solver kernel, every
+    matrix checked on the GPU and compared with the CPU oracle matrix by matrix."""
+    import numpy as np
+    import torch
+    import eigeng_batched as eg
+    import oracle
+    L, n = 1024, 32
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0xE16E0001)
+    B = torch.rand((L, n, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    a0 = (0.5 * (B + B.transpose(1, 2))).contiguous()
+    ms, _, a, w, info = time_solver(a0, 0, steps, warmup)
+    acc = accuracy_all(a0, a, w)
+    _, wo, io = oracle.eig_batch(a0.cpu().numpy(), oracle.IMTQL2)
+    wg = w.cpu().numpy()
+    scale = np.abs(wo).max(axis=1, keepdims=True)
+    par = float(np.max(np.abs(wg - wo) / scale))
+    p = eg.launch_params(L, n, 8)
+    tl = tuned_line(a0, ms, steps, warmup) if tuned else None
+    b = 8 * (n * (n + 1) / 2 + n * n + n)
+    f = (10.0 / 3.0) * n ** 3
+    gbs = L * b / (ms * 1e-3) / 1e9
+    tfs = L * f / (ms * 1e-3) / 1e12
+    del a0, a, w, B
+    torch.cuda.empty_cache()
+    return {"workload": "L=1024 n=nm=m=32 FP64, A = (B + B^T)/2, B ~ U[-1,1] (seeded)",
+            "value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms,
+            "nonconverged": int((info != 0).sum().item()) + int(io.max()),
+            "accuracy_all": acc, "oracle_max_eig_err_rel": par,
+            "launch": {k: p[k] for k in ("grpSZ", "numTB", "numGR", "numTH")},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": gbs / PEAK_HBM_GBS, "bytes_per_matrix": b,
+                         "kernel": "eig_wave_kernel<double,32> (one launch per step)",
+                         "valu_tflops": tfs, "valu_frac_fp64": tfs / PEAK_FP64_TFLOPS},
+            "tuned": tl}
+
+
 def bench_letkf48(steps, warmup):
     """The learned-warm-start workload's shape solved by the plain solver: one
     time step of L = 65536 LETKF matrices, n = 48, FP32, k = 48 members,
@@ -483,7 +522,7 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
     return {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
             "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
             "accuracy_all": acc2, "tuned": tuned2, "vendor_baseline_512": vend,
-            "roofline": {"bound": "mfma", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
+            "roofline": {"bound": "valu", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
                          "kernel": "eig_tri_large_kernel + eig_ql_lane_kernel + eig_apply_large_kernel "
@@ -516,7 +555,7 @@ def bench_n128(steps, warmup):
     return {"workload": "L=8192 n=128 FP32, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
             "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
             "accuracy_all": acc2,
-            "roofline": {"bound": "mfma", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
+            "roofline": {"bound": "valu", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
                          "kernel": "eig_tri_kernel<float,128> + eig_ql_lane_kernel + eig_apply_kernel<float,128> "
@@ -528,20 +567,55 @@ def bench_n128(steps, warmup):
                              "traffic": _traffic("traffic_eig_split128_n128_fp32.json")}}
 
 
-def cpu_baseline(a_host, min_seconds=10.0, max_reps=20):
+def host_threads():
+    """Threads for the CPU legs: the CPUs this process may run on
+    (os.sched_getaffinity), capped by OMP_NUM_THREADS when the host sets it
+    (a shared GPU host gives each GPU's jobs a share of its cores that way)."""
+    visible = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(env) if env.isdigit() and int(env) > 0 else visible
+    return min(visible, share), visible
+
+
+def time_cpu(a_host, solver, threads, min_seconds, max_reps=50):
+    """Matrices/s of the C oracle over repeated passes of a_host (bounded time)."""
     import numpy as np
     import oracle
-    threads = os.cpu_count() or 1
-    threads = min(threads, 16)
     reps, t0, total = 0, time.perf_counter(), 0
     while True:
-        _, _, info = oracle.eig_batch(a_host, oracle.IMTQL2, threads=threads)
+        _, _, info = oracle.eig_batch(a_host, solver, threads=threads)
         reps += 1
         total += a_host.shape[0]
         el = time.perf_counter() - t0
         if el >= min_seconds or reps >= max_reps:
             break
-    return total / el, threads, reps, el, int(np.count_nonzero(info))
+    return {"value": total / el, "matrices": int(total), "passes": reps, "seconds": el,
+            "nonconverged": int(np.count_nonzero(info))}
+
+
+def cpu_baseline(a_host, seconds=10.0):
+    """The reference's CPU path restated in C (oracle/): EISPACK tred1 + imtql2
+    + trbak on every thread of the host's share (one matrix per OpenMP thread),
+    the same on one thread, and the classical Jacobi of jac.cpp:60-217 on every
+    thread; each leg bounded to about `seconds` of CPU work."""
+    import oracle
+    threads, visible = host_threads()
+    L = a_host.shape[0]
+    allc = time_cpu(a_host, oracle.IMTQL2, threads, seconds)
+    one = time_cpu(a_host[: max(1, min(L, 512))], oracle.IMTQL2, 1, seconds / 2)
+    jac = time_cpu(a_host[: max(1, min(L, 4096))], oracle.JACOBI_CLASSICAL, threads, seconds / 2)
+    return {"value": allc["value"], "unit": "matrices/s", "cores": threads, "kind": "port",
+            "host_cpus_visible": visible,
+            "sample": f"{a_host.shape[0]} matrices of the same workload x {allc['passes']} passes "
+                      f"({allc['seconds']:.1f} s), C restatement of EISPACK tred1 + imtql2 + trbak "
+                      f"(oracle/), one matrix per OpenMP thread, {threads} threads",
+            "nonconverged": allc["nonconverged"],
+            "single_core": {"value": one["value"], "unit": "matrices/s", "cores": 1,
+                            "sample": f"{one['matrices']} matrices ({one['seconds']:.1f} s)"},
+            "jacobi_classical": {"value": jac["value"], "unit": "matrices/s", "cores": threads,
+                                 "sample": f"{jac['matrices']} matrices ({jac['seconds']:.1f} s), "
+                                           "classical max-pivot Jacobi of jac.cpp:60-217 restated in C",
+                                 "nonconverged": jac["nonconverged"]}}
 
 
 def accuracy_all(a0, z, w):
@@ -589,6 +663,8 @@ def main():
     ap.add_argument("--no-letkf48", action="store_true", help="skip the L=65536 n=48 line")
     ap.add_argument("--no-n128", action="store_true", help="skip the n=128 FP32 line")
     ap.add_argument("--n128-only", action="store_true", help="print only the n=128 FP32 line")
+    ap.add_argument("--no-n32", action="store_true", help="skip the n=32 FP64 L=1024 line")
+    ap.add_argument("--n32-only", action="store_true", help="print only the n=32 FP64 L=1024 line")
     ap.add_argument("--no-letkf-step", action="store_true", help="skip the LETKF analysis-step line")
     ap.add_argument("--no-vendor", action="store_true", help="skip the hipSOLVER syevjBatched comparison")
     ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned lines")
@@ -617,6 +693,9 @@ def main():
         return
     if args.n128_only:
         print(json.dumps({"n128": bench_n128(args.steps, args.warmup)}))
+        return
+    if args.n32_only:
+        print(json.dumps({"n32": bench_n32(args.steps, args.warmup, tuned=not args.no_tuned)}))
         return
     dt = torch.float32 if args.dtype == "fp32" else torch.float64
     L, n = args.L, args.n
@@ -671,7 +750,7 @@ def main():
                                f"LETKF matrices n=nm=m={n} {args.dtype}, k={k_mem} p={p_obs} "
                                f"rho={rho} sigma={sigma}",
                    "L": L, "n": n, "solver": ["imtql2", "tql2", "jacobi"][args.solver]},
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": peak_c, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": peak_c, "unit": "TFLOP/s",
                      "frac": achieved_tf / peak_c, "traffic": traffic, "kernel": kernel,
                      "kernel_ms": kern_ms, "phase_ms": phases,
                      "note": "compute roof at the reference's (10/3)n^3 flops per matrix over the "
@@ -692,6 +771,8 @@ def main():
         out["n256"] = bench_n256(3, 1, vendor=not args.no_vendor, tuned=not args.no_tuned)
     if not args.no_n128:
         out["n128"] = bench_n128(5, 1)
+    if not args.no_n32:
+        out["n32"] = bench_n32(20, 3, tuned=not args.no_tuned)
     if not args.no_letkf48:
         out["letkf48"] = bench_letkf48(5, 1)
     if not args.no_letkf_step:
@@ -701,12 +782,7 @@ def main():
         torch.cuda.empty_cache()
         out["uneven"] = bench_uneven(args.uneven_count, 8, 128, 3, 1)
     if not args.no_cpu:
-        sample = a0_host[: min(L, 16384)]
-        v, cores, reps, el, cbad = cpu_baseline(sample, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": v, "unit": "matrices/s", "cores": cores, "kind": "port",
-                               "sample": f"{sample.shape[0]} matrices of the same workload x {reps} "
-                                         f"passes ({el:.1f} s), C oracle (EISPACK tred1+imtql2+"
-                                         f"trbak restatement), one matrix per OpenMP thread"}
+        out["cpu_baseline"] = cpu_baseline(a0_host[: min(L, 16384)], args.cpu_seconds)
     print(json.dumps(out))
 
 

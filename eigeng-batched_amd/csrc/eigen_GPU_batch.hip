@@ -136,12 +136,14 @@ int global_grid(int L) { return L < 1024 ? L : 1024; }
 // small fixed block keeps the reference's "wk == NULL means query" protocol
 // yielding a non-NULL allocation); the global kernel needs one n x n
 // accumulator per resident workgroup.
-int tuned_solver(int L, int n, int tb, int solver);
-
+// Workspace of the path that `solver` (with its tuning bits) selects.  The
+// size query of every entry point reports this for the bits the caller gave,
+// never for a tuned choice: the reference ABI carries no workspace length, so
+// a remembered choice is only taken when it fits what the query reported
+// (tuned_solver).
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
   (void)nm; (void)m;
-  solver = tuned_solver(L, n, tbytes, solver);
   if (use_split(L, n, solver) || use_split128(L, n, tbytes, solver))
     return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
   if (n > GL_NMAX) return 0;
@@ -382,13 +384,24 @@ TuneKey tune_key(int L, int n, int tb, int solver)
   return TuneKey{dev, tb, n, solver & 0xff, l_bucket(L)};
 }
 
+// The tuning bits a call runs with: the caller's own bits, else a remembered
+// choice for this shape whose workspace is no larger than the one the size
+// query reports for the call (workspace_bytes of the untuned solver), else
+// none.  So a caller that sized wk before (or without) tuning never sees a
+// path that needs more.
 int tuned_solver(int L, int n, int tb, int solver)
 {
   if (solver & ~0xff) return solver;                 // the caller chose
-  std::lock_guard<std::mutex> lk(tune_mu);
-  if (tune_table.empty()) return solver;
-  auto it = tune_table.find(tune_key(L, n, tb, solver));
-  return it == tune_table.end() ? solver : (solver | it->second);
+  int bits = 0;
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    if (tune_table.empty()) return solver;
+    auto it = tune_table.find(tune_key(L, n, tb, solver));
+    if (it == tune_table.end()) return solver;
+    bits = it->second;
+  }
+  if (workspace_bytes(L, n, n, n, tb, solver | bits) > workspace_bytes(L, n, n, n, tb, solver)) return solver;
+  return solver | bits;
 }
 
 template <typename T>
@@ -486,9 +499,12 @@ hipError_t run_tune(int L, int nm, int n, const T *a_src, T *a, T *w, void *wk, 
   if (hipEventCreate(&e0) != hipSuccess) return hipErrorOutOfMemory;
   if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return hipErrorOutOfMemory; }
   const size_t bytes = (size_t)L * nm * n * sizeof(T);
+  // the workspace the default size query reports for this shape: only a
+  // candidate that fits in it may be remembered for later default calls
+  const size_t ws_default = workspace_bytes(L, nm, n, n, (int)sizeof(T), solver & 0xff);
   hipError_t err = hipSuccess;
-  int bi = -1;
-  float bms = 0.0f;
+  int bi = -1, bfit = -1;
+  float bms = 0.0f, bfit_ms = 0.0f;
   int nrep = 0;
   for (size_t i = 0; i < cands.size() && err == hipSuccess; ++i) {
     const int sv = (solver & 0xff) | cands[i];
@@ -506,6 +522,10 @@ hipError_t run_tune(int L, int nm, int n, const T *a_src, T *a, T *w, void *wk, 
     if (err != hipSuccess) break;
     if (cand && cand_ms && nrep < max_report) { cand[nrep] = cands[i]; cand_ms[nrep] = tmin; ++nrep; }
     if (bi < 0 || tmin < bms) { bi = (int)i; bms = tmin; }
+    if (workspace_bytes(L, nm, n, n, (int)sizeof(T), sv) <= ws_default && (bfit < 0 || tmin < bfit_ms)) {
+      bfit = (int)i;
+      bfit_ms = tmin;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -513,9 +533,9 @@ hipError_t run_tune(int L, int nm, int n, const T *a_src, T *a, T *w, void *wk, 
   if (err != hipSuccess) return err;
   *best = cands[bi];
   if (best_ms) *best_ms = bms;
-  {
+  if (bfit >= 0) {
     std::lock_guard<std::mutex> lk(tune_mu);
-    tune_table[tune_key(L, n, (int)sizeof(T), solver)] = cands[bi];
+    tune_table[tune_key(L, n, (int)sizeof(T), solver)] = cands[bfit];
   }
   // leave a holding the eigenvectors of the chosen configuration
   err = hipMemcpyAsync(a, a_src, bytes, hipMemcpyDeviceToDevice, s);
@@ -768,6 +788,9 @@ hipError_t run_check(int L, int nm, int n, int m, const T *a, const T *w, const 
                      hipStream_t s)
 {
   if (L < 1 || n < 1 || nm < n || m < 1 || m > n || n > GL_NMAX || !a || !w || !z) return hipErrorInvalidValue;
+  // the report (four floats) written over each a_k must fit inside a_k, or it
+  // would overwrite the next matrix (or run past the end of a)
+  if (into_a && (size_t)nm * n * sizeof(T) < 4 * sizeof(float)) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)(L < 4096 ? L : 4096);
   hipLaunchKernelGGL((eig_check_kernel<T>), dim3(grid), dim3(CK_THREADS), 0, s, L, nm, n, m, a, w, z, out, into_a);
   return hipGetLastError();
@@ -778,6 +801,11 @@ hipError_t run_check(int L, int nm, int n, int m, const T *a, const T *w, const 
 template <typename T>
 void check_entry(int L, int nm, int n, int m, T *a, T *w, T *z, hipStream_t s)
 {
+  if ((size_t)nm * n * sizeof(T) < 4 * sizeof(float)) {
+    fprintf(stderr, "eigen_GPU_check: a matrix of %d x %d elements of %d bytes cannot hold the 16-byte report; "
+            "use eigen_GPU_check_x_*\n", nm, n, (int)sizeof(T));
+    return;
+  }
   if (run_check<T>(L, nm, n, m, a, w, z, nullptr, 1, s) != hipSuccess) return;
   std::vector<float> rep((size_t)L * 4);
   const size_t pitch = (size_t)nm * n * sizeof(T);
@@ -869,6 +897,58 @@ hipError_t run_gram(int L, int nm, int n, int np, const T *y, int ldy, const T *
     hipLaunchKernelGGL((eig_gram_kernel<T, false>), dim3(grid), dim3(sd_threads(n)), 0, s, L, nm, n, np, y, ldy,
                        rinv, rstride, alpha, a);
   return hipGetLastError();
+}
+
+// Launch shape of the first kernel of the path run() takes, in the terms of
+// the reference's eigen_GPU_batch_get_Launch_params (eigen_GPU_batch.hpp:23-102):
+// p[0] lanes per matrix group, p[1] workgroups, p[2] matrices (groups) per
+// workgroup, p[3] threads per workgroup, p[4] dynamic LDS bytes passed at launch
+// (0: every kernel here sizes its LDS statically), p[5] L2 bytes of the device.
+hipError_t launch_params(int L, int n, int tb, int solver, int *p)
+{
+  for (int i = 0; i < 6; ++i) p[i] = 0;
+  if (L < 1 || n < 1 || n > GL_NMAX || (tb != 4 && tb != 8)) return hipErrorInvalidValue;
+  solver = tuned_solver(L, n, tb, solver);
+  const int base = solver & 0xff;
+  auto nb_of = [](int n_) { return n_ <= 8 ? 8 : n_ <= 16 ? 16 : n_ <= 32 ? 32 : 64; };
+  auto grp = [&](int nb, int nmat, int threads) {
+    const int g = nb < 64 ? 64 / nb : 1;
+    p[0] = nb;
+    p[2] = g;
+    p[3] = threads;
+    p[1] = (nmat + g - 1) / g;
+  };
+  if (tb == 4 && use_split128(L, n, tb, solver)) {
+    grp(128, L < LCHUNK ? L : LCHUNK, 128);
+  } else if (use_split(L, n, solver)) {
+    grp(nb_of(n), L < LCHUNK ? L : LCHUNK, 64);
+  } else if (base == SOLVER_JACOBI) {
+    if (n > 64) return hipErrorNotSupported;
+    grp(nb_of(n), L, 64);
+  } else if (n <= 64) {
+    grp(nb_of(n), L, 64);
+  } else if (use_split_large(L, n, tb, solver)) {
+    const int lc = L < LCHUNK_LARGE ? L : LCHUNK_LARGE;
+    const int tc = (solver >> TRI_CFG_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
+    const bool sym = n <= 256 && !(solver & TRI_FULL_BIT);
+    const int cfg = tc ? tc : (sym ? 1 : (n <= 256 ? 2 : 1));
+    int grid = lc < 1024 ? lc : 1024;
+    if (gc && 128 * gc < grid) grid = 128 * gc;
+    const int threads = cfg == 3 ? 1024 : cfg == 2 ? 512 : 256;
+    p[0] = threads; p[1] = grid; p[2] = 1; p[3] = threads;
+  } else if (use_wave_kernel(n, tb)) {
+    grp(128, L, 128);
+  } else {
+    const int ntc = (solver >> GL_NT_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
+    int grid = global_grid(L);
+    if (gc && 128 * gc < grid) grid = 128 * gc;
+    const int threads = ntc == 2 ? 512 : ntc == 3 ? 1024 : GL_THREADS;
+    p[0] = threads; p[1] = grid; p[2] = 1; p[3] = threads;
+  }
+  int dev = 0, l2 = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&l2, hipDeviceAttributeL2CacheSize, dev) == hipSuccess)
+    p[5] = l2;
+  return hipSuccess;
 }
 
 template <typename T>
@@ -986,6 +1066,18 @@ int eigen_GPU_batch_tune_DP(int L, int nm, int n, const double *a_src, double *a
                                ncand);
 }
 
+int eigen_GPU_batch_tune_set(int L, int n, int dtype_bytes, int solver, int bits)
+{
+  if (L < 1 || n < 1 || n > GL_NMAX || (dtype_bytes != 4 && dtype_bytes != 8) || (bits & 0xff) ||
+      (bits & (FLAG_PH_ALL | FLAG_TINYLOG)))
+    return (int)hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lk(tune_mu);
+  const TuneKey key = tune_key(L, n, dtype_bytes, solver);
+  if (bits == 0) tune_table.erase(key);
+  else tune_table[key] = bits;
+  return (int)hipSuccess;
+}
+
 void eigen_GPU_batch_tune_reset(void)
 {
   std::lock_guard<std::mutex> lk(tune_mu);
@@ -1027,6 +1119,13 @@ int eigen_GPU_batch_gram_DP(int L, int nm, int n, int p, const double *y, int ld
   return (int)run_gram<double>(L, nm, n, p, y, ldy, rinv, rstride, alpha, a, stream);
 }
 
-const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.1"; }
+int eigen_GPU_batch_launch_params(int L, int n, int dtype_bytes, int solver, int *params)
+{
+  if (!params) return (int)hipErrorInvalidValue;
+  return (int)launch_params(L, n, dtype_bytes, solver, params);
+}
+
+const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.2"; }
 
 } // extern "C"
+

@@ -85,6 +85,10 @@ def lib() -> ctypes.CDLL:
             f.restype = ci
         L.eigen_GPU_batch_tune_bufsize.argtypes = [ci, ci, ci, ci, ci]
         L.eigen_GPU_batch_tune_bufsize.restype = sz
+        L.eigen_GPU_batch_tune_set.argtypes = [ci, ci, ci, ci, ci]
+        L.eigen_GPU_batch_tune_set.restype = ci
+        L.eigen_GPU_batch_launch_params.argtypes = [ci, ci, ci, ci, vp]
+        L.eigen_GPU_batch_launch_params.restype = ci
         L.eigen_GPU_batch_tune_reset.argtypes = []
         L.eigen_GPU_batch_tune_reset.restype = None
         L.eigen_GPU_batch_uneven_bufsize.argtypes = [ci, ci, ci, ci]
@@ -338,6 +342,21 @@ def gram(y, p: int, alpha: float = 0.0, rinv=None, rstride: int = 0, nm: int | N
     _check_rc(f(L, nm, n, p, y.data_ptr(), ldy, None if rinv is None else rinv.data_ptr(), rstride, alpha,
                 a.data_ptr(), _stream_ptr(stream)), "gram")
     return a
+
+
+def tune_set(L: int, n: int, dtype_bytes: int, bits: int, solver: int = IMTQL2) -> None:
+    """Remember tuning bits for a shape (bits = 0 forgets it); see eigen_GPU_batch_tune_set."""
+    _check_rc(lib().eigen_GPU_batch_tune_set(L, n, dtype_bytes, solver, bits), "eigen_GPU_batch_tune_set")
+
+
+def launch_params(L: int, n: int, dtype_bytes: int, solver: int = IMTQL2) -> dict:
+    """Launch shape of the first kernel a call takes (eigen_GPU_batch_launch_params)."""
+    p = (ctypes.c_int * 6)()
+    rc = lib().eigen_GPU_batch_launch_params(L, n, dtype_bytes, solver, ctypes.addressof(p))
+    keys = ("grpSZ", "numTB", "numGR", "numTH", "sizeSH", "sizeL2")
+    out = dict(zip(keys, list(p)))
+    out["rc"] = rc
+    return out
 
 
 def tune_reset() -> None:

@@ -138,7 +138,10 @@ int eigen_GPU_batch_uneven_DP(int L, int nmax, const int *n, const long long *ao
 /* Replaces eigen_GPU_check_DP / _FP (reference eigen_GPU_check.h:20-23,
  * eigen_GPU_check.cu:186-198): the report (err1/max|w| or 0, err2 or 0, TOL,
  * eps, as floats) is written over the first four floats of each a_k, then read
- * back (the call synchronizes `stream`) and every failure printed to stdout. */
+ * back (the call synchronizes `stream`) and every failure printed to stdout.
+ * When a matrix's storage (nm * n elements) is smaller than the 16-byte report
+ * (FP32 n = 1 with nm <= 3, FP64 nm = n = 1) nothing is checked and a message
+ * goes to stderr; the reference writes past the matrix there. */
 void eigen_GPU_check_DP(const int L, const int nm, const int n, const int m,
                         double *a_, double *w_, double *z_, const gpuStream_t stream);
 void eigen_GPU_check_FP(const int L, const int nm, const int n, const int m,
@@ -158,11 +161,16 @@ int eigen_GPU_check_x_FP(int L, int nm, int n, int m, const float *a, const floa
  * set for (L, nm, n, element type, solver) on the caller's batch: a_src holds
  * the L input matrices (left untouched), a is a work copy that is restored from
  * a_src before each trial and ends holding the eigenvectors of the chosen set,
- * w the eigenvalues.  The winner is returned in *best (its time in *best_ms)
- * and remembered for this device: later calls with the same n, element type and
- * solver, an L in the same power-of-two range and no tuning bits of their own
- * use it (workspace queries included).  cand / cand_ms (may be NULL) receive up
- * to max_report candidates and their times, *ncand their number.  wk must hold
+ * w the eigenvalues.  The winner is returned in *best (its time in *best_ms;
+ * pass it as tuning bits to the _x entry points, with a workspace sized by
+ * eigen_GPU_batch_bufsize for those bits).  For this device the library
+ * remembers the fastest candidate whose workspace fits in what the default
+ * size query reports for the shape: later calls with the same n, element type
+ * and solver, an L in the same power-of-two range and no tuning bits of their
+ * own run it.  Size queries never change through tuning, so a workspace sized
+ * before tuning stays large enough (the reference ABI carries no workspace
+ * length).  cand / cand_ms (may be NULL) receive up to max_report candidates
+ * and their times, *ncand their number.  wk must hold
  * eigen_GPU_batch_tune_bufsize(...) bytes.  Synchronizes stream.
  */
 size_t eigen_GPU_batch_tune_bufsize(int L, int nm, int n, int dtype_bytes, int solver);
@@ -172,6 +180,13 @@ int eigen_GPU_batch_tune_FP(int L, int nm, int n, const float *a_src, float *a, 
 int eigen_GPU_batch_tune_DP(int L, int nm, int n, const double *a_src, double *a, double *w,
                             void *wk, hipStream_t stream, int solver, int *best, float *best_ms,
                             int max_report, int *cand, float *cand_ms, int *ncand);
+/* Remember tuning bits for (n, element size, solver, the power-of-two range of
+ * L) on the current device, e.g. a choice saved from an earlier tuning run;
+ * bits == 0 forgets the entry.  The same rule as for the tuner's own choices
+ * holds: later calls without bits run them only when their workspace fits in
+ * what the default size query reports.  Returns hipSuccess or
+ * hipErrorInvalidValue (bad sizes, solver bits in `bits`, phase or test bits). */
+int eigen_GPU_batch_tune_set(int L, int n, int dtype_bytes, int solver, int bits);
 /* Forget every tuned choice (all devices). */
 void eigen_GPU_batch_tune_reset(void);
 
@@ -212,6 +227,18 @@ int eigen_GPU_batch_gram_FP(int L, int nm, int n, int p, const float *y, int ldy
 int eigen_GPU_batch_gram_DP(int L, int nm, int n, int p, const double *y, int ldy,
                             const double *rinv, int rstride, double alpha, double *a,
                             hipStream_t stream);
+
+/*
+ * Launch shape of the first kernel of the path a call with (L, n, element
+ * size, solver) takes (tuning included), in the terms of the reference's
+ * eigen_GPU_batch_get_Launch_params (eigen_GPU_batch.hpp:23-102): params[0]
+ * lanes per matrix group, [1] workgroups, [2] matrices (groups) per
+ * workgroup, [3] threads per workgroup, [4] dynamic LDS bytes passed at
+ * launch (0: the kernels size their LDS statically), [5] L2 bytes of the
+ * current device.  Returns hipSuccess, or hipErrorInvalidValue /
+ * hipErrorNotSupported with params zeroed.
+ */
+int eigen_GPU_batch_launch_params(int L, int n, int dtype_bytes, int solver, int *params);
 
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);

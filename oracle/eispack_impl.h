@@ -310,10 +310,92 @@ static int FN(jacobi)(int nm, int n, REAL *a, REAL *z, REAL *w, REAL tol, int ma
   return 1;
 }
 
-/* Solver ids (shared with the GPU library's extended entry points). */
+/* Classical Jacobi of the reference's CPU prototype jac.cpp:60-217.  Column
+ * l keeps the row maxpos[l] of its largest off-diagonal |a(k,l)| and that
+ * value absele[l] (jac.cpp:82-92); each step takes the column j with the
+ * largest absele and rotates the pair (i = maxpos[j], j) (find_maxpos,
+ * jac.cpp:40-55).  The rotation is jac.cpp's: t from tau = beta/alpha or
+ * alpha/beta by the larger of |alpha| = |2 a_ij| and |beta| = |a_jj - a_ii|
+ * (jac.cpp:116-129), c = 1/sqrt(1+t^2), s = c t; w(i), w(j) and a(i,j) as
+ * jac.cpp:143-146; rows/columns i, j of A and columns i, j of Z rotated
+ * (jac.cpp:151-177); column maxima raised as jac.cpp:188-212.  Stops when the
+ * pivot repeats (jac.cpp:105) or when absele[i] < (|a_ii| + |a_jj|) tol
+ * (jac.cpp:114).  Two guards are added: when every tracked maximum is zero
+ * (the matrix is diagonal: jac.cpp would then index row 0), and after
+ * max_rot rotations (returns 1).  Eigenvalues unsorted in w. */
+static int FN(jacobi_classical)(int nm, int n, REAL *a, REAL *z, REAL *w, REAL tol, long max_rot, int *maxpos,
+                                REAL *absele)
+{
+  FN(zero_identity)(nm, n, z);
+  for (int l = 1; l <= n; ++l) w[l - 1] = A_(l, l);
+  for (int l = 1; l <= n; ++l) {
+    int pos = 0;
+    REAL amax = 0;
+    for (int k = 1; k <= n; ++k)
+      if (k != l && FN(fabs_)(A_(k, l)) > amax) { amax = FN(fabs_)(A_(k, l)); pos = k; }
+    maxpos[l - 1] = pos;
+    absele[l - 1] = amax;
+  }
+  int i = 0, j = 0;
+  for (long itr = 0; itr < max_rot; ++itr) {
+    const int i0 = i, j0 = j;
+    int jm = 1;
+    REAL b = 0;
+    for (int k = 1; k <= n; ++k)
+      if (b < absele[k - 1]) { b = absele[k - 1]; jm = k; }
+    if (b == (REAL)0) return 0;
+    i = maxpos[jm - 1];
+    j = jm;
+    if (i0 == i && j0 == j) return 0;
+    REAL aij = A_(i, j);
+    const REAL bij = absele[i - 1], aii = w[i - 1], ajj = w[j - 1];
+    if (bij < (FN(fabs_)(aii) + FN(fabs_)(ajj)) * tol) return 0;
+    const REAL alpha = 2 * aij, beta = ajj - aii;
+    REAL t;
+    if (FN(fabs_)(alpha) >= FN(fabs_)(beta)) {
+      const REAL tau = beta / alpha, r = FN(sqrt_)(1 + tau * tau);
+      t = 1 / (tau + (tau >= 0 ? r : -r));
+    } else {
+      const REAL tau = alpha / beta, r = FN(sqrt_)(1 + tau * tau);
+      t = tau / (1 + r);
+    }
+    const REAL c = 1 / FN(sqrt_)(1 + t * t), sn = c * t;
+    w[i - 1] = (c * c) * aii - 2 * (c * sn) * aij + (sn * sn) * ajj;
+    w[j - 1] = (sn * sn) * aii + 2 * (c * sn) * aij + (c * c) * ajj;
+    aij = (c * sn) * (aii - ajj) + (c - sn) * (c + sn) * aij;
+    A_(i, j) = aij;
+    A_(j, i) = aij;
+    absele[i - 1] = absele[j - 1] = FN(fabs_)(aij);
+    for (int k = 1; k <= n; ++k) {
+      const REAL zki = Z_(k, i), zkj = Z_(k, j);
+      Z_(k, i) = c * zki - sn * zkj;
+      Z_(k, j) = sn * zki + c * zkj;
+      if (k != i && k != j) {
+        const REAL aki = A_(k, i), akj = A_(k, j);
+        const REAL bik = c * aki - sn * akj, bjk = sn * aki + c * akj;
+        A_(i, k) = A_(k, i) = bik;
+        A_(j, k) = A_(k, j) = bjk;
+        if (maxpos[k - 1] == i) absele[k - 1] = FN(fabs_)(bik);
+        if (maxpos[k - 1] == j) absele[k - 1] = FN(fabs_)(bjk);
+      }
+    }
+    for (int k = 1; k <= n; ++k) {
+      if (k == i || k == j) continue;
+      const REAL bik = FN(fabs_)(A_(i, k)), bjk = FN(fabs_)(A_(j, k));
+      if (absele[i - 1] < bik) { maxpos[i - 1] = k; absele[i - 1] = bik; }
+      if (absele[j - 1] < bjk) { maxpos[j - 1] = k; absele[j - 1] = bjk; }
+      if (absele[k - 1] < bik) { maxpos[k - 1] = i; absele[k - 1] = bik; }
+      if (absele[k - 1] < bjk) { maxpos[k - 1] = j; absele[k - 1] = bjk; }
+    }
+  }
+  return 1;
+}
+
+/* Solver ids (shared with the GPU library's extended entry points; 3 is the
+ * CPU-only classical Jacobi). */
 #ifndef ORACLE_SOLVERS
 #define ORACLE_SOLVERS
-enum { ORC_IMTQL2 = 0, ORC_TQL2 = 1, ORC_JACOBI = 2 };
+enum { ORC_IMTQL2 = 0, ORC_TQL2 = 1, ORC_JACOBI = 2, ORC_JACOBI_CLASSICAL = 3 };
 #endif
 
 /* One matrix, full pipeline as eigen_GPU_batch does (eigen_GPU_batch.cu:93-134):
@@ -328,6 +410,16 @@ static int FN(eig1)(int nm, int n, REAL *a, REAL *w, int solver, REAL *z, REAL *
   const REAL eps = (sizeof(REAL) == 8) ? (REAL)DBL_EPSILON : (REAL)FLT_EPSILON;
   const REAL tol = eps * (REAL)((sizeof(REAL) == 8) ? 512 : 16);
   if (n == 1) { w[0] = a[0]; a[0] = 1; return 0; }
+  if (solver == ORC_JACOBI_CLASSICAL) {
+    /* jac.cpp reads the full symmetric matrix: mirror the upper triangle.
+     * Tolerance eps^2 as jac.cpp's driver (jac.cpp:227-228). */
+    for (int c = 1; c <= n; ++c) for (int r = c + 1; r <= n; ++r) A_(r, c) = A_(c, r);
+    const int err = FN(jacobi_classical)(nm, n, a, z, d, eps * eps, 200L * n * n, pos, tmp);
+    FN(sort_pos)(n, d, pos);
+    for (int j = 0; j < n; ++j) w[j] = d[j];
+    for (int c = 1; c <= n; ++c) for (int r = 1; r <= n; ++r) A_(r, c) = Z_(r, pos[c - 1] + 1);
+    return err;
+  }
   if (solver == ORC_JACOBI) {
     /* Jacobi reads the full symmetric matrix: mirror the upper triangle. */
     for (int c = 1; c <= n; ++c) for (int r = c + 1; r <= n; ++r) A_(r, c) = A_(c, r);

@@ -21,6 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 IMTQL2, TQL2, JACOBI = 0, 1, 2
+# CPU only: the classical (max-pivot) Jacobi of the reference's jac.cpp:60-217
+JACOBI_CLASSICAL = 3
 _lib = None
 
 

@@ -90,7 +90,12 @@ int main() {
   gpuError_t e3 = eigen_GPU_batch_BufferSize<float>(16, 8, 9, 9, (float*)0, (float*)0, &lw_bad);
   size_t q = 0;
   eigen_GPU_batch_FP(16, 64, 64, 64, 0, (float*)&q, 0, 0);
-  std::printf("%d %d %d %zu %zu %zu %zu\n", (int)e1, (int)e2, (int)e3, lw_f, lw_d, lw_bad, q);
+  int g = -1, tb = -1, gr = -1, th = -1, sh = -1, l2 = -1;
+  gpuError_t e4 = eigen_GPU_batch_get_Launch_params<float>(16384, 64, g, tb, gr, th, sh, l2);
+  int g2 = -1, tb2 = -1, gr2 = -1, th2 = -1, sh2 = -1, l22 = -1;
+  gpuError_t e5 = eigen_GPU_batch_get_Launch_params<double>(0, 64, g2, tb2, gr2, th2, sh2, l22);
+  std::printf("%d %d %d %zu %zu %zu %zu %d %d %d %d %d %d %d %d\n", (int)e1, (int)e2, (int)e3, lw_f, lw_d,
+              lw_bad, q, (int)e4, g, tb, gr, th, sh, (int)e5, g2 + tb2 + gr2 + th2 + sh2 + l22);
   return 0;
 }
 ''')
@@ -100,10 +105,70 @@ int main() {
            "-o", str(exe), f"-L{PKG}", "-leigenGbatch", f"-Wl,-rpath,{PKG}"]
     subprocess.run(cmd, check=True, capture_output=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    e1, e2, e3, lw_f, lw_d, lw_bad, q = map(int, out)
+    e1, e2, e3, lw_f, lw_d, lw_bad, q, e4, g, tb, gr, th, sh, e5, zero = map(int, out)
     assert e1 == 0 and e2 == 0 and e3 != 0
     assert lw_f == eg.bufsize(16, 64, 64, 64, 4) and lw_d == eg.bufsize(16, 33, 33, 33, 8)
     assert lw_bad == 0 and q == lw_f
+    # get_Launch_params<T>: the split path's first kernel at L = 16384, n = 64
+    # (one 64-lane group per matrix, one matrix per one-wave workgroup)
+    assert e4 == 0 and (g, tb, gr, th, sh) == (64, 16384, 1, 64, 0)
+    assert e5 != 0 and zero == 0       # invalid L: every output zeroed, as the reference does
+
+
+def test_launch_params_follow_the_dispatch():
+    """eigen_GPU_batch_launch_params reports the first kernel of the path a
+    call takes: the fused register kernel for small batches (groups of 8..64
+    lanes, 64 / group-size matrices per wave), the split path from 2048
+    matrices, the two-wave FP32 kernels up to n = 128, the workgroup kernels
+    beyond."""
+    eg.tune_reset()
+    p = eg.launch_params(1024, 32, 8)          # n = 32 FP64 L = 1024: fused, 2 matrices per wave
+    assert p["rc"] == 0 and (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (32, 512, 2, 64)
+    p = eg.launch_params(100, 5, 4)
+    assert (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (8, 13, 8, 64)
+    p = eg.launch_params(8192, 128, 4)         # FP32 n = 128, L >= 2048: two-wave register split
+    assert (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (128, 8192, 1, 128)
+    p = eg.launch_params(8192, 256, 8)         # FP64 n = 256: large split, 256-thread tridiagonalisation
+    assert (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (256, 1024, 1, 256)
+    p = eg.launch_params(100, 300, 8)          # small batch beyond 256: workgroup kernel
+    assert (p["numTB"], p["numGR"], p["numTH"]) == (100, 1, 256)
+    for bad in ((0, 10, 4), (10, 0, 4), (10, 513, 8), (10, 10, 2)):
+        p = eg.launch_params(*bad)
+        assert p["rc"] != 0 and all(p[k] == 0 for k in ("grpSZ", "numTB", "numGR", "numTH"))
+
+
+@pytest.mark.parametrize("tb", [4, 8])
+def test_tuning_never_changes_the_size_query(tb):
+    """A remembered tuning choice never needs more workspace than the default
+    size query reports (the reference ABI carries no workspace length): the
+    query is the same before and after, and a choice that does not fit is not
+    taken (the launch shape stays the default one), while one that fits is."""
+    eg.tune_reset()
+    try:
+        L, n = 1000, 48                         # default: fused (256 bytes); split needs tens of MB
+        q0 = eg.query_c_abi(L, n, n, n, tb)
+        b0 = eg.bufsize(L, n, n, n, tb)
+        p0 = eg.launch_params(L, n, tb)
+        assert eg.bufsize(L, n, n, n, tb, eg.FLAG_SPLIT) > q0
+        eg.tune_set(L, n, tb, eg.FLAG_SPLIT)
+        assert eg.query_c_abi(L, n, n, n, tb) == q0 and eg.bufsize(L, n, n, n, tb) == b0
+        assert eg.launch_params(L, n, tb) == p0, "a tuned path that needs more workspace was taken"
+        # n = 200, L = 4096: default the large split path (256-thread
+        # tridiagonalisation); the 512-thread workgroup kernel needs less
+        L2, n2, wg512 = 4096, 200, eg.FLAG_FUSED | (2 << 16)
+        q2 = eg.query_c_abi(L2, n2, n2, n2, tb)
+        assert eg.launch_params(L2, n2, tb)["numTH"] == 256
+        assert eg.bufsize(L2, n2, n2, n2, tb, wg512) < q2
+        eg.tune_set(L2, n2, tb, wg512)
+        assert eg.query_c_abi(L2, n2, n2, n2, tb) == q2
+        p2 = eg.launch_params(L2, n2, tb)
+        assert (p2["grpSZ"], p2["numTB"], p2["numGR"], p2["numTH"]) == (512, 1024, 1, 512)
+        eg.tune_set(L2, n2, tb, 0)              # forget it again
+        assert eg.launch_params(L2, n2, tb)["numTH"] == 256
+        with pytest.raises(RuntimeError):
+            eg.tune_set(L2, n2, tb, eg.FLAG_SPLIT | 0x1000)    # phase bits cannot be remembered
+    finally:
+        eg.tune_reset()
 
 
 @pytest.mark.parametrize("sfx", ["FP", "DP"])

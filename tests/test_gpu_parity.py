@@ -205,7 +205,8 @@ def test_large_upper_triangle_only(n, dtype, path):
 def test_large_split_letkf_and_qdq(dtype):
     """n = 256 clustered spectra (LETKF-like, and pairs split by 1e-8) through
     the large split path."""
-    for a0 in (wl.letkf_y(6, k=256, p=400, rho=1.05, sigma=0.1, dtype=dtype, seed=5)[0],):
+    qdq = wl.qdq(wl.clustered_uniform(6, 256, frac=0.05, gap=1e-8, seed=6), dtype=dtype, seed=6)
+    for a0 in (wl.letkf_y(6, k=256, p=400, rho=1.05, sigma=0.1, dtype=dtype, seed=5)[0], qdq):
         zo, wo, _ = oracle.eig_batch(a0, oracle.IMTQL2)
         z, w, info = run_gpu(a0, path="split")
         assert info.max() == 0

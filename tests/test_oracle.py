@@ -31,7 +31,7 @@ def solve(a, solver=oracle.IMTQL2):
     return z, w
 
 
-@pytest.mark.parametrize("solver", [oracle.IMTQL2, oracle.TQL2, oracle.JACOBI])
+@pytest.mark.parametrize("solver", [oracle.IMTQL2, oracle.TQL2, oracle.JACOBI, oracle.JACOBI_CLASSICAL])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 8, 16, 31, 32, 33, 47, 64, 65, 100, 128])
 def test_eigenvalues_vs_lapack(n, dtype, solver):
@@ -210,3 +210,24 @@ def test_workload_generators():
     assert np.allclose(np.linalg.eigvalsh(wl.unpack(q)), np.sort(lam0, axis=1), atol=1e-12)
     ls = wl.logspaced_shuffled(2, 10)
     assert np.isclose(ls.max(), 1.0) and np.isclose(ls.min(), 1e-6)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_classical_jacobi_special_cases(dtype):
+    """The classical (jac.cpp) Jacobi: a diagonal matrix needs no rotation
+    (the guard for all-zero column maxima), the identity keeps Z = I, the
+    2 x 2 closed form, and the Frank closed form."""
+    n = 9
+    d = np.array([3.0, -1.0, 2.0, 2.0, 0.5, 7.0, -4.0, 0.0, 1.0])
+    a = np.broadcast_to(np.diag(d), (1, n, n)).astype(dtype).copy()
+    z, w = solve(a, oracle.JACOBI_CLASSICAL)
+    assert np.array_equal(w[0], np.sort(d).astype(dtype))
+    assert np.array_equal(np.abs(z[0]), np.abs(z[0]).round())           # a permutation
+    a2 = np.array([[[2.0, 1.0], [1.0, 2.0]]], dtype=dtype)              # eigenvalues 1, 3
+    z, w = solve(a2, oracle.JACOBI_CLASSICAL)
+    assert np.allclose(w[0], [1.0, 3.0], rtol=4 * eps_of(dtype))
+    assert np.allclose(np.abs(z[0]), np.sqrt(0.5), rtol=8 * eps_of(dtype))
+    af = wl.frank(1, 24, 24, dtype=dtype)
+    _, w = solve(af, oracle.JACOBI_CLASSICAL)
+    lam = wl.frank_eigenvalues(24)
+    assert np.max(np.abs(w - lam) / lam[-1]) <= 64 * 24 * eps_of(dtype)
