@@ -1129,3 +1129,5 @@ const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.2"; }
 
 } // extern "C"
 
+// the no-allocation entry points (scratch array in device memory)
+#include "eig_noalloc.hpp"

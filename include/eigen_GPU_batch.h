@@ -240,6 +240,27 @@ int eigen_GPU_batch_gram_DP(int L, int nm, int n, int p, const double *y, int ld
  */
 int eigen_GPU_batch_launch_params(int L, int n, int dtype_bytes, int solver, int *params);
 
+/*
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The computation of eigen_GPU_batch_FP_x /
+ * _DP_x (same kernels) with the workspace taken from a fixed device array owned
+ * by the library: eigen_GPU_batch_noalloc_scratch_bytes() = 256 MiB on each
+ * device that runs it, held until the process ends.  A batch whose workspace
+ * exceeds the array is solved in consecutive sub-batches (halving the count
+ * until it fits; a sub-batch may take another path than the whole batch would,
+ * so results can then differ from the _x call's at rounding level).  Returns
+ * hipErrorInvalidValue for bad sizes, phase or test bits, or when one matrix's
+ * workspace exceeds the array; hipErrorNotSupported for n > 512.  Nothing is
+ * allocated and no memset is issued; the array needs no clearing between
+ * calls.  Calls that overlap on one device race on the array.
+ */
+size_t eigen_GPU_batch_noalloc_scratch_bytes(void);
+int eigen_GPU_batch_FP_noalloc(int L, int nm, int n, int m, float *a, float *w, hipStream_t stream,
+                               int solver, int *info);
+int eigen_GPU_batch_DP_noalloc(int L, int nm, int n, int m, double *a, double *w, hipStream_t stream,
+                               int solver, int *info);
+
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);
 
