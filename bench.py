@@ -513,6 +513,7 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
     torch.cuda.empty_cache()
     tuned2 = tuned_line(q0, ms2, steps, warmup) if tuned else None
     vend = vendor_baseline(q0[:512].contiguous(), reps=1) if vendor else None
+    mixed = bench_mixed(q0, ms2, steps, warmup)
     ph2, a2, w2, _ = time_phases(q0, 2, 1)
     del q0, a2, w2
     torch.cuda.empty_cache()
@@ -521,7 +522,7 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
     b2 = 8 * (n2 * (n2 + 1) / 2 + n2 * n2 + n2)
     return {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
             "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
-            "accuracy_all": acc2, "tuned": tuned2, "vendor_baseline_512": vend,
+            "accuracy_all": acc2, "tuned": tuned2, "vendor_baseline_512": vend, "mixed": mixed,
             "roofline": {"bound": "valu", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
@@ -532,6 +533,56 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
             "roofline_hbm": {"achieved": L2 * b2 / (kms2 * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                              "unit": "GB/s", "bytes_per_matrix": b2,
                              "traffic": _traffic("traffic_eig_split_n256_fp64.json")}}
+
+
+def gemm_f64_roofline(batch, n, reps=5):
+    """HIP-event time of one FP64 matrix-core product of the refinement's shape
+    (batch x (n x n) x (n x n), eig_gemm_kernel<GK_F64>), on the stream it is
+    launched on: achieved TFLOP/s at 2 n^3 flops per product."""
+    import numpy as np
+    import torch
+    import eigeng_batched as eg
+    x = torch.randn((batch, n, n), dtype=torch.float64, device="cuda")
+    y = torch.randn((batch, n, n), dtype=torch.float64, device="cuda")
+    c = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    eg.gemm(x, y, c=c, stream=stream)
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        eg.gemm(x, y, c=c, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ms = float(np.mean(ts))
+    tf = batch * 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+    del x, y, c
+    return {"kernel": "eig_gemm_kernel<GK_F64> (v_mfma_f64_16x16x4f64)", "batch": batch, "n": n,
+            "ms": ms, "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS}
+
+
+def bench_mixed(q0, plain_ms, steps, warmup):
+    """The FP64 interface's mixed-precision solver (FP32 solve, FP64 refinement on
+    the matrix cores, plain fallback) on the same batch: time, its ratio to the
+    plain FP64 solve, every matrix checked on the GPU (relative residual
+    ||AZ - ZD||_F / max|w| and orthogonality), and the FP32 solve alone."""
+    import torch
+    import eigeng_batched as eg
+    L, n, _ = q0.shape
+    ms, _, a, w, info = time_solver(q0, eg.MIXED, steps, warmup)
+    acc = accuracy_all(q0, a, w)
+    bad = int((info != 0).sum().item())
+    del a, w
+    torch.cuda.empty_cache()
+    q32 = q0.float().contiguous()
+    ms32, _, a32, w32, _ = time_solver(q32, 0, steps, warmup)
+    del q32, a32, w32
+    torch.cuda.empty_cache()
+    return {"value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms, "vs_plain": plain_ms / ms,
+            "fp32_solve_ms": ms32, "refinement_ms": ms - ms32, "nonconverged": bad, "accuracy_all": acc,
+            "relative_residual_max": acc["max_err1_rel"],
+            "gemm_roofline": gemm_f64_roofline(2048, n)}
 
 
 def bench_n128(steps, warmup):
@@ -671,6 +722,8 @@ def main():
     ap.add_argument("--letkf-step-only", action="store_true", help="print only the LETKF analysis-step line")
     ap.add_argument("--n256-only", action="store_true",
                     help="run only the n=256 FP64 line (for profiling its kernels)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="print only the n=256 FP64 mixed-precision line (and the plain time)")
     ap.add_argument("--uneven-count", type=int, default=100000)
     args = ap.parse_args()
 
@@ -684,6 +737,13 @@ def main():
     import torch
     import eigeng_batched as eg
 
+    if args.mixed_only:
+        import torch
+        q0 = make_qdq_gpu(8192, 256, seed=0xE16E0004, dtype=torch.float64)
+        plain_ms, _, a2, w2, _ = time_solver(q0, 0, args.steps, args.warmup)
+        del a2, w2
+        print(json.dumps({"mixed": bench_mixed(q0, plain_ms, args.steps, args.warmup), "plain_ms": plain_ms}))
+        return
     if args.n256_only:
         print(json.dumps({"n256": bench_n256(args.steps, args.warmup, vendor=not args.no_vendor,
                                              tuned=not args.no_tuned)}))

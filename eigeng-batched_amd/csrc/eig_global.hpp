@@ -190,7 +190,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
     for (;;) {
       if (wv == 0) {
         int status = 0, m = l;                 // 0: converged, 1: sweep, 2: error
-        if (solver == SOLVER_TQL2) {
+        if (kopt_solver(solver) == SOLVER_TQL2) {
           if (!tql_started) {
             const T h = (abs_(S.d[l]) + abs_(S.e[l])) * tol;
             tst1 = (h > tst1) ? h : tst1;
@@ -226,7 +226,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
           }
         }
         if (status == 1) {
-          if (solver == SOLVER_TQL2) {
+          if (kopt_solver(solver) == SOLVER_TQL2) {
             const T dl_old = S.d[l], el = S.e[l];
             const T p0 = (S.d[l + 1] - dl_old) / (el + el);
             const T r0 = hypot1_c(p0);
@@ -304,7 +304,7 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
       tql_started = true;
       __syncthreads();
     }
-    if (!err && solver == SOLVER_TQL2) {
+    if (!err && kopt_solver(solver) == SOLVER_TQL2) {
       if (t == 0) S.d[l] += shift;
       __syncthreads();
     }
@@ -322,7 +322,8 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
   if (wv == 0) {
     for (int i = lane; i < n; i += 64) S.pos[i] = i;
     wsync();
-    for (int l = 0; l < n - 1; ++l) {
+    const int nsort = kopt_unsorted(solver) ? 0 : n - 1;
+    for (int l = 0; l < nsort; ++l) {
       T v = T(0);
       int ix = GL_NMAX;
       for (int idx = l + lane; idx < n; idx += 64) {
@@ -348,7 +349,8 @@ __device__ __forceinline__ void global_solve(T *__restrict__ A, int nm, int n, T
   }
   __syncthreads();
   for (int i = t; i < n; i += NT) wout[i] = S.d[i];
-  for (int j = 0; j < n; ++j) {
+  const int mcols = n - kopt_mdrop(solver);
+  for (int j = 0; j < mcols; ++j) {
     const int src = S.pos[j];
     for (int r = t; r < n; r += NT) A[r + (size_t)j * nm] = Z[r + (size_t)src * n];
   }

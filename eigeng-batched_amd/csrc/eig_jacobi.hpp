@@ -12,7 +12,8 @@
 // b = a_qq - a_pp, c = 1/hypot(1, t), s = c t; then A <- J^T A J (columns
 // lane-locally, rows by exchanging lanes p and q) and Z <- Z J.  A rotation is
 // skipped when |a_pq| <= eps (|a_pp| + |a_qq|); the iteration stops after a
-// sweep with no rotation or after JAC_MAX_SWEEP sweeps (info = -1 then).
+// sweep with no rotation or after `maxsw` sweeps (JAC_MAX_SWEEP for the solver
+// entry points; info = -1 then), and nsweeps (if given) gets the count.
 // The reference pivots one rotation at a time on the largest element
 // (jac_tiled.hpp:33-64); the cyclic order converges to the same eigenpairs.
 #pragma once
@@ -31,7 +32,9 @@ struct alignas(16) JacLds {
 
 template <typename T, int NB>
 __global__ __launch_bounds__(64) void eig_jacobi_kernel(int L, int nm, int n, T *__restrict__ a,
-                                                        T *__restrict__ w, int *__restrict__ info)
+                                                        T *__restrict__ w, int *__restrict__ info,
+                                                        int opts = 0, int maxsw = JAC_MAX_SWEEP,
+                                                        int *__restrict__ nsweeps = nullptr)
 {
   constexpr int G = 64 / NB;
   constexpr int H = NB / 2;
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(64) void eig_jacobi_kernel(int L, int nm, int n, T 
   const T eps = eps_of<T>();
 
   int sweep = 0;
-  for (; sweep < JAC_MAX_SWEEP; ++sweep) {
+  for (; sweep < maxsw; ++sweep) {
     bool rotated = false;
     for (int round = 0; round < NB - 1; ++round) {
       // own diagonal a_kk and the pair element a_{k, NB-1-k}
@@ -130,7 +133,8 @@ __global__ __launch_bounds__(64) void eig_jacobi_kernel(int L, int nm, int n, T 
   T dk = T(0);
 #pragma unroll
   for (int j = 0; j < NB; ++j) dk = (k == j) ? RA[j] : dk;
-  if (info && k == 0) info[mat] = (sweep >= JAC_MAX_SWEEP) ? -1 : 0;
+  if (info && k == 0) info[mat] = (sweep >= maxsw) ? -1 : 0;
+  if (nsweeps && k == 0) nsweeps[mat] = sweep;
   // rank = number of real positions (orig < n) ordered before this one by
   // (value, original index); the reference's Jacobi leaves its output unsorted
   // (jac_tiled.hpp:143-147), the ABI returns ascending eigenvalues for every solver
@@ -142,6 +146,7 @@ __global__ __launch_bounds__(64) void eig_jacobi_kernel(int L, int nm, int n, T 
     const bool before = (oj < n) && (dj < dk || (dj == dk && oj < orig));
     rank += before ? 1 : 0;
   }
+  if (kopt_unsorted(opts)) rank = orig;           // the rotations' order (the reference's)
   if (real) {
     w[(size_t)mat * n + rank] = dk;
     S.pos[k] = rank;
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(64) void eig_jacobi_kernel(int L, int nm, int n, T 
     const int oj = lane_bcast(orig, base + j);
     if (oj < n) {
       const int r = S.pos[j];
-      if (k < n) A[k + (size_t)r * nm] = RZ[j];
+      if (k < n && r < n - kopt_mdrop(opts)) A[k + (size_t)r * nm] = RZ[j];
     }
   }
 }

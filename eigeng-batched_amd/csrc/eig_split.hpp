@@ -158,7 +158,7 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
   const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
   int nrot = 0, nsw = 0, st = 0;
 
-  if (solver == SOLVER_TQL2) {
+  if (kopt_solver(solver) == SOLVER_TQL2) {
     // explicit shift (tql2_tiled.hpp:40-146), as one flat loop of sweeps per
     // lane like the implicit-shift form below: l is set up (tst1, m; the
     // converged ones finished) when its predecessor converges, every pass
@@ -287,7 +287,8 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
   }
   // selection sort with the reference's rule; eS now holds the positions
   for (int i = 0; i < n; ++i) eS[i][lane] = T(i);
-  for (int l = 0; l < n - 1; ++l) {
+  const int nsort = kopt_unsorted(solver) ? 0 : n - 1;
+  for (int l = 0; l < nsort; ++l) {
     T dl = dS[l][lane];
     int il = l;
     for (int j = l + 1; j < n; ++j) {
@@ -302,9 +303,11 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
       dS[l][lane] = dl;
     }
   }
+  // output position of each column; the columns beyond m are not stored (-1)
+  const int mcols = n - kopt_mdrop(solver);
   for (int i = 0; i < n; ++i) {
     wout[i] = dS[i][lane];
-    sout[(int)eS[i][lane]] = i;
+    sout[(int)eS[i][lane]] = i < mcols ? i : -1;
   }
 }
 
@@ -347,7 +350,7 @@ template <typename T, int NB>
 __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
     int L, int n, const T *__restrict__ de, int ld, typename vec2<T>::type *__restrict__ rotlog,
     int rot_cap, int *__restrict__ swlog, int sw_cap, int *__restrict__ nsw_out, int *__restrict__ status,
-    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info)
+    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int opts = 0)
 {
   typedef typename vec2<T>::type T2;
   static_assert(NB <= 64, "one 64-bit mask per lane");
@@ -493,9 +496,10 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
   // selection sort with the reference's rule; e now holds the positions
 #pragma unroll
   for (int i = 0; i < NB; ++i) e[i] = T(i);
+  const int nsort = kopt_unsorted(opts) ? 0 : n - 1;
 #pragma unroll
   for (int l2 = 0; l2 < NB - 1; ++l2) {
-    if (l2 < n - 1) {
+    if (l2 < nsort) {
       T dm = d[l2], pm = e[l2];
       int im = l2;
 #pragma unroll
@@ -516,11 +520,12 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
       e[l2] = pm;
     }
   }
+  const int mcols = n - kopt_mdrop(opts);
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     if (i < n) {
       w[(size_t)mat * n + i] = d[i];
-      slot[(size_t)mat * n + (int)e[i]] = i;
+      slot[(size_t)mat * n + (int)e[i]] = i < mcols ? i : -1;
     }
   }
 }
@@ -625,7 +630,7 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
   for (int j = 0; j < NB; ++j) {
     if (j < n) {
       const int r = S.pos[j];
-      if (k < n) A[k + (size_t)r * nm] = R[j];
+      if (k < n && r >= 0) A[k + (size_t)r * nm] = R[j];
     }
   }
 }

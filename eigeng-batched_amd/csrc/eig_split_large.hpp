@@ -759,7 +759,7 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
   if (r < n) {
 #pragma unroll
     for (int j = 0; j < CW; ++j)
-      if (c0 + j < n) A[r + (size_t)S.pos[q][j] * nm] = x[j];
+      if (c0 + j < n && S.pos[q][j] >= 0) A[r + (size_t)S.pos[q][j] * nm] = x[j];
   }
 }
 

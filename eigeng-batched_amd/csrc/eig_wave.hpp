@@ -32,7 +32,19 @@ namespace eigmi {
 
 enum { SOLVER_IMTQL2 = 0, SOLVER_TQL2 = 1, SOLVER_JACOBI = 2 };
 constexpr int QL_MAX_SWEEP = 100; // tql2_tiled.hpp:7
-constexpr int ST_OVERFLOW = -2;   // split path: rotation log full, solve with the fused kernel
+constexpr int ST_OVERFLOW = -2;
+// Output options packed with the solver id into the kernels' `solver`
+// argument: bits 0-7 the solver, bit 8 keep the QL / Jacobi order of the
+// eigenpairs (the reference's DO_SORT 0, eigen_GPU_batch.cu:5,10), bits 12-21
+// the number of trailing eigenvector columns not written (n - m).  Launches
+// that pass a bare solver id get the default: sorted, all n columns.
+__host__ __device__ constexpr int kopt_make(int solver, bool unsorted, int mdrop)
+{
+  return (solver & 0xff) | (unsorted ? 0x100 : 0) | ((mdrop & 0x3ff) << 12);
+}
+__host__ __device__ constexpr int kopt_solver(int o) { return o & 0xff; }
+__host__ __device__ constexpr bool kopt_unsorted(int o) { return ((o >> 8) & 1) != 0; }
+__host__ __device__ constexpr int kopt_mdrop(int o) { return (o >> 12) & 0x3ff; }   // split path: rotation log full, solve with the fused kernel
 
 // Offset of packed Householder column c (1 <= c) when column c' holds
 // roundup8(c') entries: sum_{c'=1}^{c-1} 8*ceil(c'/8).
@@ -300,7 +312,7 @@ __device__ __forceinline__ void wave_solve(T *__restrict__ A, int nm, int n, T *
     for (int h = 0; h < NW; ++h) { csc[h] = T(1); css[h] = T(0); }
   };
 
-  if (solver == SOLVER_TQL2) {
+  if (kopt_solver(solver) == SOLVER_TQL2) {
     // explicit shift (tql2_tiled.hpp:40-146)
     T shift = T(0), tst1 = T(0);
     for (int l = 0; l < n && !err; ++l) {
@@ -447,7 +459,8 @@ __device__ __forceinline__ void wave_solve(T *__restrict__ A, int nm, int n, T *
   int pv[NW];
 #pragma unroll
   for (int q = 0; q < NW; ++q) pv[q] = kl + q * NL;
-  for (int l = 0; l < n - 1; ++l) {
+  const int nsort = kopt_unsorted(solver) ? 0 : n - 1;
+  for (int l = 0; l < nsort; ++l) {
     // this lane's best candidate over its slots (ties to the smaller index)
     T v = T(0);
     int ix = NB;
@@ -488,11 +501,12 @@ __device__ __forceinline__ void wave_solve(T *__restrict__ A, int nm, int n, T *
     }
   }
   wsync();
+  const int mcols = n - kopt_mdrop(solver);
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (j < n) {
       const int r = posw[j];
-      if (k < n) A[k + (size_t)r * nm] = R[j];
+      if (k < n && r < mcols) A[k + (size_t)r * nm] = R[j];
     }
   }
 }

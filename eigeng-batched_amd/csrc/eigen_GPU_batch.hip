@@ -19,6 +19,10 @@
 #include "eig_split_large.hpp"
 #include "eig_check.hpp"
 #include "eig_symdmt.hpp"
+#include "eig_gemm.hpp"
+#include "eig_refine.hpp"
+#include "eig_layout.hpp"
+#include "eig_warm.hpp"
 
 #include <stdio.h>
 #include <vector>
@@ -52,6 +56,24 @@ constexpr int FLAG_PH_TRI = 0x1000, FLAG_PH_QL = 0x2000, FLAG_PH_APPLY = 0x4000;
 // lane-per-matrix QL, 64-row-block apply) instead of the large-order one
 constexpr int FLAG_REG128 = 0x8000;
 constexpr int FLAG_PH_ALL = FLAG_PH_TRI | FLAG_PH_QL | FLAG_PH_APPLY;
+// eigenpairs in the QL / Jacobi order instead of ascending (the reference's
+// DO_SORT 0, eigen_GPU_batch.cu:5,10)
+constexpr int FLAG_UNSORTED = 0x40000000;
+// FP64 interface, mixed precision: FP32 solve of the rounded matrices, then
+// FP64 refinement on MFMA (eig_refine.hpp) with the plain FP64 solver as the
+// fallback for matrices the refinement does not accept
+constexpr int SOLVER_MIXED = 3;
+// bits 28-29 of `solver`: refinement iterations (0: the default, 3)
+constexpr int MIX_IT_SHIFT = 28;
+constexpr int MIX_DEFAULT_ITERS = 3;
+// refinement chunk (matrices whose FP64 work arrays are live at once)
+constexpr int LCHUNK_REF = 2048;
+// accepted when the last iteration's relative measures before the update are at
+// most this (the update squares them).  From an FP32 start the first update
+// leaves second-order terms of up to ~1e-5 (pairs just outside a cluster), the
+// second ~1e-10, so with the default three iterations the bound is a check on
+// the second update's result, and the third brings everything to FP64 level.
+constexpr double MIX_ACCEPT = 1e-6;
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -141,9 +163,56 @@ int global_grid(int L) { return L < 1024 ? L : 1024; }
 // never for a tuned choice: the reference ABI carries no workspace length, so
 // a remembered choice is only taken when it fits what the query reported
 // (tuned_solver).
+size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver);
+
+// FP64 refinement (eig_refine.hpp) of a chunk of lc matrices: six n x n FP64
+// arrays (A, P = A X then E, X, X', S, G), the statistics step's per-matrix
+// results, status, reasons and the largest first-order entry.
+struct RefineLayout {
+  size_t af, pe, x0, x1, sm, gm, inf, st, why, emx, total;
+  int lc;
+  RefineLayout(int L, int n)
+  {
+    lc = L < LCHUNK_REF ? L : LCHUNK_REF;
+    const size_t nn = (size_t)n * n;
+    af = 0;
+    pe = af + al256((size_t)lc * nn * 8);
+    x0 = pe + al256((size_t)lc * nn * 8);
+    x1 = x0 + al256((size_t)lc * nn * 8);
+    sm = x1 + al256((size_t)lc * nn * 8);
+    gm = sm + al256((size_t)lc * nn * 8);
+    inf = gm + al256((size_t)lc * nn * 8);
+    st = inf + al256((size_t)lc * rf_info_bytes(n));
+    why = st + al256((size_t)lc * 4);
+    emx = why + al256((size_t)lc * 4);
+    total = emx + al256((size_t)lc * 8);
+  }
+};
+
+// Mixed-precision path (FP64 interface): the FP32 copy of the batch, its
+// eigenvalues and status, the FP32 solver's own workspace, then the refinement
+// region.
+struct MixedLayout {
+  size_t a32, w32, info32, ws32, ref, total;
+  MixedLayout(int L, int n)
+  {
+    const size_t nn = (size_t)n * n;
+    a32 = 0;
+    w32 = a32 + al256((size_t)L * nn * 4);
+    info32 = w32 + al256((size_t)L * n * 4);
+    ws32 = info32 + al256((size_t)L * 4);
+    ref = ws32 + al256(workspace_bytes(L, n, n, n, 4, 0));
+    total = ref + RefineLayout(L, n).total;
+  }
+};
+
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
   (void)nm; (void)m;
+  if ((solver & 0xff) == SOLVER_MIXED) {
+    if (tbytes != 8 || n > 256) return 0;
+    return WS_HEAD + MixedLayout(L, n).total;
+  }
   if (use_split(L, n, solver) || use_split128(L, n, tbytes, solver))
     return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
   if (n > GL_NMAX) return 0;
@@ -154,18 +223,19 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 }
 
 template <typename T, int NB>
-hipError_t launch_wave(int L, int nm, int n, T *a, T *w, int *info, int solver, hipStream_t s)
+hipError_t launch_wave(int L, int nm, int n, T *a, T *w, int *info, int kop, hipStream_t s)
 {
   constexpr int G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
   const long long blocks = ((long long)L + G - 1) / G;
   if (blocks > 0x7fffffffll) return hipErrorInvalidValue;
   hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3((unsigned)blocks), dim3(64 * NW), 0, s,
-                     L, nm, n, a, w, info, solver & 0xff, (const int *)nullptr);
+                     L, nm, n, a, w, info, kop, (const int *)nullptr);
   return hipGetLastError();
 }
 
 template <typename T, int NB>
-hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
+hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver, int kop, void *wk,
+                        hipStream_t s)
 {
   typedef typename vec2<T>::type T2;
   constexpr int G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
@@ -192,29 +262,29 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
       const bool reg = (solver & 0xff) == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS) && NB * sizeof(T) <= 256;
       if (reg)
         hipLaunchKernelGGL((eig_ql_reg_kernel<T, (NB * sizeof(T) <= 256 ? NB : 8)>), dim3(lb), dim3(64), 0, s,
-                           cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic);
+                           cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, kop);
       else
         hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap,
-                           sw, scap, nsw, st, slot, wc, ic, solver & 0xff);
+                           sw, scap, nsw, st, slot, wc, ic, kop);
     }
     if (ph & FLAG_PH_APPLY) {
       // NB = 128: one workgroup per 64-row block (NW per matrix)
       hipLaunchKernelGGL((eig_apply_kernel<T, NB>), dim3(gb * NW), dim3(64), 0, s, cur, nm, n, ac, q, rot, rcap,
                          sw, scap, nsw, st, slot);
       hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3(gb), dim3(64 * NW), 0, s, cur, nm, n, ac, wc, ic,
-                         solver & 0xff, (const int *)st);
+                         kop, (const int *)st);
     }
   }
   return hipGetLastError();
 }
 
 template <typename T, int NB>
-hipError_t launch_jacobi(int L, int nm, int n, T *a, T *w, int *info, hipStream_t s)
+hipError_t launch_jacobi(int L, int nm, int n, T *a, T *w, int *info, int kop, hipStream_t s)
 {
   constexpr int G = 64 / NB;
   const long long blocks = ((long long)L + G - 1) / G;
   if (blocks > 0x7fffffffll) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((eig_jacobi_kernel<T, NB>), dim3((unsigned)blocks), dim3(64), 0, s, L, nm, n, a, w, info);
+  hipLaunchKernelGGL((eig_jacobi_kernel<T, NB>), dim3((unsigned)blocks), dim3(64), 0, s, L, nm, n, a, w, info, kop);
   return hipGetLastError();
 }
 
@@ -233,13 +303,14 @@ constexpr int TRI_CFG_SHIFT = 24;
 constexpr int TRI_FULL_BIT = 1 << 26;
 
 template <typename T>
-hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
+hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver, int kop, void *wk,
+                         hipStream_t s)
 {
   T *zws = (T *)((char *)wk + WS_HEAD);
   const int ntc = (solver >> GL_NT_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
   int grid = global_grid(L);
   if (gc) grid = (128 * gc < grid) ? 128 * gc : grid;
-  const int base = solver & 0xff;
+  const int base = kop;
   switch (ntc) {
   case 2:
     hipLaunchKernelGGL((eig_global_kernel<T, 512>), dim3(grid), dim3(512), 0, s, L, nm, n, a, w, info, base, zws,
@@ -266,7 +337,8 @@ void launch_apply_large(unsigned grid, int L, int nm, int n, T *a, const T *Wm, 
 }
 
 template <typename T>
-hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int solver, void *wk, hipStream_t s)
+hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int solver, int kop, void *wk,
+                              hipStream_t s)
 {
   typedef typename vec2<T>::type T2;
   const int lc = L < LCHUNK_LARGE ? L : LCHUNK_LARGE;
@@ -277,7 +349,7 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
   int *sw = (int *)(base + lay.sw), *nsw = (int *)(base + lay.nsw);
   int *st = (int *)(base + lay.status), *slot = (int *)(base + lay.slot);
   const int rcap = (solver & FLAG_TINYLOG) ? 1 : rot_cap_of(n), scap = sw_cap_of(n);
-  const int sv = solver & 0xff;
+  const int sv = kop;
   const int ph = (solver & FLAG_PH_ALL) ? (solver & FLAG_PH_ALL) : FLAG_PH_ALL;
   if (ph != FLAG_PH_ALL && L > lc) return hipErrorInvalidValue;
   for (long long c0 = 0; c0 < L; c0 += lc) {
@@ -352,6 +424,198 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
   return hipGetLastError();
 }
 
+// ---- batched products on the matrix cores ---------------------------------------
+
+bool overlap(const void *a, size_t na, const void *b, size_t nb);
+
+size_t gemm_extent(long long rows, long long cols, long long ld, long long stride, int batch)
+{
+  return (size_t)((batch - 1) * stride + (cols - 1) * ld + rows);
+}
+
+// Validates and launches eig_gemm_kernel<kind>: one workgroup per 64 x 64 tile.
+hipError_t gemm_launch(int kind, const GemmArgs &g, hipStream_t s)
+{
+  if (kind < GK_F64 || kind > GK_BF16) return hipErrorInvalidValue;
+  if (g.M < 1 || g.N < 1 || g.K < 1 || g.batch < 0 || !g.A || !g.B || !g.C) return hipErrorInvalidValue;
+  if (g.batch == 0) return hipSuccess;
+  const long long ar = g.transa ? g.K : g.M, ac = g.transa ? g.M : g.K;
+  const long long br = g.transb ? g.N : g.K, bc = g.transb ? g.K : g.N;
+  if (g.lda < ar || g.ldb < br || g.ldc < g.M || (g.D && g.ldd < g.M)) return hipErrorInvalidValue;
+  if (g.sa < 0 || g.sb < 0 || g.sc < 0 || g.sd < 0) return hipErrorInvalidValue;
+  const int eb = kind == GK_F64 ? 8 : 4;
+  const size_t ea = gemm_extent(ar, ac, g.lda, g.sa, g.batch) * eb;
+  const size_t ebb = gemm_extent(br, bc, g.ldb, g.sb, g.batch) * eb;
+  const size_t ec = gemm_extent(g.M, g.N, g.ldc, g.sc, g.batch) * eb;
+  if (overlap(g.A, ea, g.C, ec) || overlap(g.B, ebb, g.C, ec)) return hipErrorInvalidValue;
+  if (g.D && g.D != g.C && overlap(g.D, gemm_extent(g.M, g.N, g.ldd, g.sd, g.batch) * eb, g.C, ec))
+    return hipErrorInvalidValue;
+  if (g.D == g.C && g.ldd != g.ldc) return hipErrorInvalidValue;
+  const long long tm = (g.M + GM_TILE - 1) / GM_TILE, tn = (g.N + GM_TILE - 1) / GM_TILE;
+  long long tiles = tm * tn;
+  if (g.sym) {
+    if (g.sym != tm || tn % g.sym) return hipErrorInvalidValue;
+    tiles = (tn / g.sym) * ((long long)g.sym * (g.sym + 1) / 2);
+  }
+  const long long total = tiles * g.batch;
+  if (total > 0x7fffffffll) return hipErrorInvalidValue;
+  switch (kind) {
+  case GK_F64: hipLaunchKernelGGL((eig_gemm_kernel<GK_F64>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
+  case GK_F32: hipLaunchKernelGGL((eig_gemm_kernel<GK_F32>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
+  case GK_F16: hipLaunchKernelGGL((eig_gemm_kernel<GK_F16>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
+  default: hipLaunchKernelGGL((eig_gemm_kernel<GK_BF16>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
+  }
+  return hipGetLastError();
+}
+
+// square n x n products of a batch of packed (ld n) FP64 matrices
+hipError_t gemm_sq64(int batch, int n, int ta, const double *A, const double *B, double beta, const double *D,
+                     double *C, int sym, hipStream_t s)
+{
+  const long long nn = (long long)n * n;
+  GemmArgs g{};
+  g.M = g.N = g.K = n;
+  g.batch = batch;
+  g.transa = ta;
+  g.transb = 0;
+  g.A = A; g.lda = n; g.sa = nn;
+  g.B = B; g.ldb = n; g.sb = nn;
+  g.D = D; g.ldd = n; g.sd = D ? nn : 0;
+  g.C = C; g.ldc = n; g.sc = nn;
+  g.alpha = 1.0;
+  g.beta = beta;
+  g.sym = sym ? (n + GM_TILE - 1) / GM_TILE : 0;
+  return gemm_launch(GK_F64, g, s);
+}
+
+template <typename T>
+hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s, int solver, int *info);
+
+// The refinement iterations on one chunk (RefineLayout at rb): the full
+// matrices are in af, X in x0, status / reasons initialised.  Returns the
+// array holding the refined X.
+hipError_t refine_chunk(int cur, int n, char *rb, const RefineLayout &ly, int iters, hipStream_t s, double **xout)
+{
+  double *af = (double *)(rb + ly.af), *pe = (double *)(rb + ly.pe);
+  double *x = (double *)(rb + ly.x0), *xn = (double *)(rb + ly.x1);
+  double *sm = (double *)(rb + ly.sm), *gm = (double *)(rb + ly.gm);
+  char *inf = rb + ly.inf;
+  int *st = (int *)(rb + ly.st), *why = (int *)(rb + ly.why);
+  unsigned long long *emx = (unsigned long long *)(rb + ly.emx);
+  const int nt = (n + 63) / 64;
+  const long long fill = (long long)cur * nt * nt;
+  hipError_t err;
+  for (int it = 0; it < iters; ++it) {
+    if ((err = gemm_sq64(cur, n, 0, af, x, 0.0, nullptr, pe, 0, s)) != hipSuccess) return err;   // P = A X
+    if ((err = gemm_sq64(cur, n, 1, x, pe, 0.0, nullptr, sm, 1, s)) != hipSuccess) return err;   // S = X^T P
+    if ((err = gemm_sq64(cur, n, 1, x, x, 0.0, nullptr, gm, 1, s)) != hipSuccess) return err;    // G = X^T X
+    hipLaunchKernelGGL(eig_oa_stats_kernel, dim3((unsigned)(cur < 8192 ? cur : 8192)), dim3(RF_THREADS), 0, s, cur,
+                       n, (const double *)sm, (const double *)gm, inf, ly.lc, st, why, emx);
+    hipLaunchKernelGGL(eig_oa_fill_kernel, dim3((unsigned)(fill < (1 << 20) ? fill : (1 << 20))), dim3(256), 0, s,
+                       cur, n, (const double *)sm, (const double *)gm, inf, ly.lc, pe, emx);
+    if ((err = gemm_sq64(cur, n, 0, x, pe, 1.0, x, xn, 0, s)) != hipSuccess) return err;        // X' = X + X E
+    double *tmp = x; x = xn; xn = tmp;
+  }
+  *xout = x;
+  return hipGetLastError();
+}
+
+// Mixed precision for the FP64 interface: A rounded to FP32 (full symmetric,
+// ld n), the FP32 solver on it (its default path and tuned choice), then per
+// chunk the FP64 refinement (eig_refine.hpp), the store, and the plain FP64
+// workgroup solver on the matrices the refinement did not accept.
+hipError_t run_mixed(int L, int nm, int n, int m, double *a, double *w, void *wk, hipStream_t s, int solver,
+                     int *info)
+{
+  if (n > 256) return hipErrorNotSupported;
+  const int iters = ((solver >> MIX_IT_SHIFT) & 3) ? ((solver >> MIX_IT_SHIFT) & 3) : MIX_DEFAULT_ITERS;
+  const int sorted = (solver & FLAG_UNSORTED) ? 0 : 1;
+  const MixedLayout lay(L, n);
+  const RefineLayout ly(L, n);
+  char *base = (char *)wk + WS_HEAD;
+  char *rb = base + lay.ref;
+  float *a32 = (float *)(base + lay.a32), *w32 = (float *)(base + lay.w32);
+  int *info32 = (int *)(base + lay.info32);
+  double *af = (double *)(rb + ly.af), *x0 = (double *)(rb + ly.x0), *sm = (double *)(rb + ly.sm);
+  const double *lnew = (const double *)(rb + ly.inf) + (size_t)ly.lc * n;   // refined eigenvalues (RfInfo::lnew)
+  int *st = (int *)(rb + ly.st), *why = (int *)(rb + ly.why);
+  const unsigned long long *emx = (const unsigned long long *)(rb + ly.emx);
+  const size_t nn = (size_t)n * n;
+  const int nt = (n + 63) / 64, tiles = nt * (nt + 1) / 2;
+  auto grid_of = [](long long work, long long cap) { return (unsigned)(work < cap ? work : cap); };
+  hipLaunchKernelGGL((eig_symfull_kernel<double, float>), dim3(grid_of((long long)L * tiles, 65536)), dim3(256), 0,
+                     s, L, n, (const double *)a, (long long)nm, (long long)nm * n, a32, (long long)n, (long long)nn);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = run<float>(L, n, n, n, a32, w32, base + lay.ws32, s, SOLVER_IMTQL2, info32);
+  if (err != hipSuccess) return err;
+  for (long long c0 = 0; c0 < L; c0 += ly.lc) {
+    const int cur = (int)((L - c0) < ly.lc ? (L - c0) : ly.lc);
+    double *ac = a + (size_t)c0 * nm * n, *wc = w + (size_t)c0 * n;
+    int *ic = info ? info + c0 : nullptr;
+    hipLaunchKernelGGL((eig_symfull_kernel<double, double>), dim3(grid_of((long long)cur * tiles, 65536)), dim3(256),
+                       0, s, cur, n, (const double *)ac, (long long)nm, (long long)nm * n, af, (long long)n,
+                       (long long)nn);
+    hipLaunchKernelGGL((eig_convert_kernel<float, double>), dim3(grid_of(((long long)cur * nn + 255) / 256, 16384)),
+                       dim3(256), 0, s, (long long)cur * (long long)nn, (const float *)(a32 + (size_t)c0 * nn), x0);
+    hipLaunchKernelGGL(eig_refine_init_kernel, dim3((cur + 255) / 256), dim3(256), 0, s, cur,
+                       (const int *)(info32 + c0), st, why);
+    double *x = nullptr;
+    if ((err = refine_chunk(cur, n, rb, ly, iters, s, &x)) != hipSuccess) return err;
+    hipLaunchKernelGGL(eig_refine_store_kernel, dim3(grid_of(cur, 4096)), dim3(RF_THREADS), 0, s, cur, nm, n, m,
+                       (const double *)x, lnew, st, why, emx, MIX_ACCEPT, ac, wc, ic, sorted, 0);
+    // the plain FP64 solver on the matrices left for it (S holds its accumulators)
+    hipLaunchKernelGGL((eig_global_kernel<double, GL_THREADS>), dim3(grid_of(cur, 256)), dim3(GL_THREADS), 0, s, cur,
+                       nm, n, ac, wc, ic, kopt_make(SOLVER_IMTQL2, !sorted, n - m), sm, (const int *)st);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+// Refinement of given approximate eigenvectors (FP64): a holds the matrices
+// (upper triangle, ld nm), x the approximate eigenvectors (ld ldx), refined in
+// place with the eigenvalues into w, ascending; info[k] = 0 when accepted,
+// else the reasons (RF_WHY_*) and x, w of that matrix hold the last iterate.
+hipError_t run_refine(int L, int nm, int n, const double *a, double *x, int ldx, double *w, void *wk, hipStream_t s,
+                      int iters, int *info)
+{
+  if (L < 1 || n < 1 || nm < n || ldx < n || !a || !x || !w || !wk) return hipErrorInvalidValue;
+  if (n > 256) return hipErrorNotSupported;
+  if (iters < 1 || iters > 8) return hipErrorInvalidValue;
+  const RefineLayout ly(L, n);
+  char *rb = (char *)wk + WS_HEAD;
+  double *af = (double *)(rb + ly.af), *x0 = (double *)(rb + ly.x0);
+  const double *lnew = (const double *)(rb + ly.inf) + (size_t)ly.lc * n;
+  int *st = (int *)(rb + ly.st), *why = (int *)(rb + ly.why);
+  const unsigned long long *emx = (const unsigned long long *)(rb + ly.emx);
+  const size_t nn = (size_t)n * n;
+  const int nt = (n + 63) / 64, tiles = nt * (nt + 1) / 2;
+  auto grid_of = [](long long work, long long cap) { return (unsigned)(work < cap ? work : cap); };
+  hipError_t err = hipSuccess;
+  for (long long c0 = 0; c0 < L; c0 += ly.lc) {
+    const int cur = (int)((L - c0) < ly.lc ? (L - c0) : ly.lc);
+    const double *ac = a + (size_t)c0 * nm * n;
+    double *xc = x + (size_t)c0 * ldx * n, *wc = w + (size_t)c0 * n;
+    int *ic = info ? info + c0 : nullptr;
+    hipLaunchKernelGGL((eig_symfull_kernel<double, double>), dim3(grid_of((long long)cur * tiles, 65536)), dim3(256),
+                       0, s, cur, n, ac, (long long)nm, (long long)nm * n, af, (long long)n, (long long)nn);
+    // X (ld ldx) -> x0 (ld n): the same tile copy with a full square source
+    hipLaunchKernelGGL((eig_copy_cols_kernel<double>), dim3(grid_of((long long)cur * n, 65536)), dim3(256), 0, s,
+                       cur, n, (const double *)xc, (long long)ldx, x0);
+    hipLaunchKernelGGL(eig_refine_init_kernel, dim3((cur + 255) / 256), dim3(256), 0, s, cur, (const int *)nullptr,
+                       st, why);
+    double *xr = nullptr;
+    if ((err = refine_chunk(cur, n, rb, ly, iters, s, &xr)) != hipSuccess) return err;
+    hipLaunchKernelGGL(eig_refine_store_kernel, dim3(grid_of(cur, 4096)), dim3(RF_THREADS), 0, s, cur, ldx, n, n,
+                       (const double *)xr, lnew, st, why, emx, MIX_ACCEPT, xc, wc, ic, 1, 1);
+    if (ic)
+      hipLaunchKernelGGL(eig_refine_info_kernel, dim3((cur + 255) / 256), dim3(256), 0, s, cur, (const int *)st,
+                         (const int *)why, ic);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
 // ---- autotuned launch shapes ------------------------------------------------
 // Tuning bits found by eigen_GPU_batch_tune_* per (device, element size, n,
 // solver, power-of-two bucket of L); used when a call brings no tuning bits.
@@ -391,7 +655,7 @@ TuneKey tune_key(int L, int n, int tb, int solver)
 // path that needs more.
 int tuned_solver(int L, int n, int tb, int solver)
 {
-  if (solver & ~0xff) return solver;                 // the caller chose
+  if (solver & ~0xff & ~FLAG_UNSORTED) return solver;  // the caller chose the path
   int bits = 0;
   {
     std::lock_guard<std::mutex> lk(tune_mu);
@@ -410,36 +674,43 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (bad_args(L, nm, n, m)) return hipErrorInvalidValue;
   solver = tuned_solver(L, n, (int)sizeof(T), solver);
   const int base = solver & 0xff;
+  if (base == SOLVER_MIXED) {
+    if (wk == nullptr) return hipErrorInvalidValue;
+    if constexpr (sizeof(T) == 8) return run_mixed(L, nm, n, m, a, w, wk, s, solver, info);
+    return hipErrorNotSupported;
+  }
   if (base != SOLVER_IMTQL2 && base != SOLVER_TQL2 && base != SOLVER_JACOBI) return hipErrorNotSupported;
   if (wk == nullptr) return hipErrorInvalidValue;
+  // kernel options: solver, order of the output, eigenvector columns not stored
+  const int kop = kopt_make(base, (solver & FLAG_UNSORTED) != 0, n - m);
   if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver) && !use_split_large(L, n, (int)sizeof(T), solver) &&
       !use_split128(L, n, (int)sizeof(T), solver))
     return hipErrorInvalidValue;
   if constexpr (sizeof(T) == 4) {
-    if (use_split128(L, n, (int)sizeof(T), solver)) return launch_split<T, 128>(L, nm, n, a, w, info, solver, wk, s);
+    if (use_split128(L, n, (int)sizeof(T), solver)) return launch_split<T, 128>(L, nm, n, a, w, info, solver, kop, wk, s);
   }
   if (use_split(L, n, solver)) {
-    if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, wk, s);
-    if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, wk, s);
-    if (n <= 32) return launch_split<T, 32>(L, nm, n, a, w, info, solver, wk, s);
-    return launch_split<T, 64>(L, nm, n, a, w, info, solver, wk, s);
+    if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, kop, wk, s);
+    if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, kop, wk, s);
+    if (n <= 32) return launch_split<T, 32>(L, nm, n, a, w, info, solver, kop, wk, s);
+    return launch_split<T, 64>(L, nm, n, a, w, info, solver, kop, wk, s);
   }
   if (base == SOLVER_JACOBI) {
-    if (n <= 8) return launch_jacobi<T, 8>(L, nm, n, a, w, info, s);
-    if (n <= 16) return launch_jacobi<T, 16>(L, nm, n, a, w, info, s);
-    if (n <= 32) return launch_jacobi<T, 32>(L, nm, n, a, w, info, s);
-    if (n <= 64) return launch_jacobi<T, 64>(L, nm, n, a, w, info, s);
+    if (n <= 8) return launch_jacobi<T, 8>(L, nm, n, a, w, info, kop, s);
+    if (n <= 16) return launch_jacobi<T, 16>(L, nm, n, a, w, info, kop, s);
+    if (n <= 32) return launch_jacobi<T, 32>(L, nm, n, a, w, info, kop, s);
+    if (n <= 64) return launch_jacobi<T, 64>(L, nm, n, a, w, info, kop, s);
     return hipErrorNotSupported;
   }
-  if (n <= 8) return launch_wave<T, 8>(L, nm, n, a, w, info, solver, s);
-  if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, solver, s);
-  if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, solver, s);
-  if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, solver, s);
+  if (n <= 8) return launch_wave<T, 8>(L, nm, n, a, w, info, kop, s);
+  if (n <= 16) return launch_wave<T, 16>(L, nm, n, a, w, info, kop, s);
+  if (n <= 32) return launch_wave<T, 32>(L, nm, n, a, w, info, kop, s);
+  if (n <= 64) return launch_wave<T, 64>(L, nm, n, a, w, info, kop, s);
   if (n > GL_NMAX) return hipErrorNotSupported;
-  if (use_split_large(L, n, (int)sizeof(T), solver)) return launch_split_large<T>(L, nm, n, a, w, info, solver, wk, s);
+  if (use_split_large(L, n, (int)sizeof(T), solver)) return launch_split_large<T>(L, nm, n, a, w, info, solver, kop, wk, s);
   if (use_wave_kernel(n, (int)sizeof(T)))
-    return launch_wave<T, (sizeof(T) == 4 ? 128 : 64)>(L, nm, n, a, w, info, solver, s);
-  return launch_global<T>(L, nm, n, a, w, info, solver, wk, s);
+    return launch_wave<T, (sizeof(T) == 4 ? 128 : 64)>(L, nm, n, a, w, info, kop, s);
+  return launch_global<T>(L, nm, n, a, w, info, solver, kop, wk, s);
 }
 
 // ---- autotuner --------------------------------------------------------------
@@ -779,6 +1050,159 @@ hipError_t run_uneven(int L, int nmax, const int *nv, const long long *aoff, con
       run_uneven_split64<T>(L, cc, list, nv, ao, wo, a, w, info, solver, ws + lay.split, zws, s);
   }
   return hipGetLastError();
+}
+
+// ---- warm-started solve (n <= 64) ------------------------------------------------
+// Workspace per chunk of lc matrices: the full matrix, later B = Q^T A Q and its
+// Jacobi eigenvectors (one region), Q, A Q then V = Q Z_B (one region), the
+// Jacobi status and the acceptance status.
+constexpr int LCHUNK_WARM = 16384;
+struct WarmLayout {
+  size_t ab, q, v, ji, st, total;
+  int lc;
+  WarmLayout(int L, int n, int tb)
+  {
+    lc = L < LCHUNK_WARM ? L : LCHUNK_WARM;
+    const size_t mat = (size_t)lc * n * n * tb;
+    ab = 0;
+    q = al256(mat);
+    v = q + al256(mat);
+    ji = v + al256(mat);
+    st = ji + al256((size_t)lc * 4);
+    total = st + al256((size_t)lc * 4);
+  }
+};
+
+size_t warm_bytes(int L, int n, int tb)
+{
+  if (L < 1 || n < 1 || n > 64 || (tb != 4 && tb != 8)) return 0;
+  return WS_HEAD + WarmLayout(L, n, tb).total;
+}
+
+template <typename T, int NB>
+void launch_jacobi_capped(int L, int n, T *b, T *w, int *jinfo, int maxsw, int *nsweeps, hipStream_t s)
+{
+  constexpr int G = 64 / NB;
+  hipLaunchKernelGGL((eig_jacobi_kernel<T, NB>), dim3((unsigned)((L + G - 1) / G)), dim3(64), 0, s, L, n, n, b, w,
+                     jinfo, kopt_make(SOLVER_JACOBI, false, 0), maxsw, nsweeps);
+}
+
+template <typename T, int NB>
+void launch_wave_only(int L, int nm, int n, T *a, T *w, int *info, int kop, const int *only, hipStream_t s)
+{
+  constexpr int G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
+  hipLaunchKernelGGL((eig_wave_kernel<T, NB>), dim3((unsigned)((L + G - 1) / G)), dim3(64 * NW), 0, s, L, nm, n, a,
+                     w, info, kop, only);
+}
+
+// q0: L approximate eigenbases in a's layout (ld nm).  Per chunk: full A, Q =
+// qr(q0), A Q, B = Q^T (A Q) (matrix cores: FP32 operands for FP32, FP64 for
+// FP64), Jacobi on B with at most maxsw sweeps, V = Q Z_B, then V stored for the
+// accepted matrices and the plain solver run on the others.
+template <typename T>
+hipError_t run_warm(int L, int nm, int n, int m, T *a, const T *q0, T *w, void *wk, hipStream_t s, int maxsw,
+                    int *info, int *sweeps)
+{
+  if (bad_args(L, nm, n, m) || !a || !q0 || !w || !wk) return hipErrorInvalidValue;
+  if (n > 64) return hipErrorNotSupported;
+  if (maxsw < 1 || maxsw > JAC_MAX_SWEEP) return hipErrorInvalidValue;
+  const int tb = (int)sizeof(T);
+  const WarmLayout lay(L, n, tb);
+  char *base = (char *)wk + WS_HEAD;
+  T *ab = (T *)(base + lay.ab), *q = (T *)(base + lay.q), *v = (T *)(base + lay.v);
+  int *ji = (int *)(base + lay.ji), *st = (int *)(base + lay.st);
+  const size_t nn = (size_t)n * n;
+  const int kind = tb == 8 ? GK_F64 : GK_F32;
+  const int kop = kopt_make(SOLVER_IMTQL2, false, n - m);
+  hipError_t err = hipSuccess;
+  for (long long c0 = 0; c0 < L; c0 += lay.lc) {
+    const int cur = (int)((L - c0) < lay.lc ? (L - c0) : lay.lc);
+    T *ac = a + (size_t)c0 * nm * n, *wc = w + (size_t)c0 * n;
+    const T *qc = q0 + (size_t)c0 * nm * n;
+    int *ic = info ? info + c0 : nullptr, *sc = sweeps ? sweeps + c0 : nullptr;
+    hipLaunchKernelGGL((eig_symfull_kernel<T, T>), dim3((unsigned)(cur < 65536 ? cur : 65536)), dim3(256), 0, s, cur,
+                       n, (const T *)ac, (long long)nm, (long long)nm * n, ab, (long long)n, (long long)nn);
+    hipLaunchKernelGGL((eig_qr_kernel<T>), dim3((unsigned)(cur < 65536 ? cur : 65536)), dim3(64), 0, s, cur, nm, n,
+                       qc, q, (long long)n, (long long)nn);
+    GemmArgs g{};
+    g.M = g.N = g.K = n;
+    g.batch = cur;
+    g.lda = g.ldb = g.ldc = n;
+    g.sa = g.sb = g.sc = (long long)nn;
+    g.alpha = 1.0;
+    g.A = ab; g.B = q; g.C = v;                      // A Q
+    if ((err = gemm_launch(kind, g, s)) != hipSuccess) return err;
+    g.transa = 1; g.A = q; g.B = v; g.C = ab;        // B = Q^T (A Q)
+    if ((err = gemm_launch(kind, g, s)) != hipSuccess) return err;
+    if (n <= 8) launch_jacobi_capped<T, 8>(cur, n, ab, wc, ji, maxsw, sc, s);
+    else if (n <= 16) launch_jacobi_capped<T, 16>(cur, n, ab, wc, ji, maxsw, sc, s);
+    else if (n <= 32) launch_jacobi_capped<T, 32>(cur, n, ab, wc, ji, maxsw, sc, s);
+    else launch_jacobi_capped<T, 64>(cur, n, ab, wc, ji, maxsw, sc, s);
+    g.transa = 0; g.A = q; g.B = ab; g.C = v;        // V = Q Z_B
+    if ((err = gemm_launch(kind, g, s)) != hipSuccess) return err;
+    hipLaunchKernelGGL((eig_warm_store_kernel<T>), dim3((unsigned)(cur < 16384 ? cur : 16384)), dim3(256), 0, s, cur,
+                       nm, n, m, (const T *)v, (const int *)ji, st, ac, ic);
+    if (n <= 8) launch_wave_only<T, 8>(cur, nm, n, ac, wc, ic, kop, st, s);
+    else if (n <= 16) launch_wave_only<T, 16>(cur, nm, n, ac, wc, ic, kop, st, s);
+    else if (n <= 32) launch_wave_only<T, 32>(cur, nm, n, ac, wc, ic, kop, st, s);
+    else launch_wave_only<T, 64>(cur, nm, n, ac, wc, ic, kop, st, s);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t run_qr(int L, int nm, int n, const T *a, T *q, hipStream_t s)
+{
+  if (L < 0 || n < 1 || n > 64 || nm < n || !a || !q) return hipErrorInvalidValue;
+  if (L == 0) return hipSuccess;
+  hipLaunchKernelGGL((eig_qr_kernel<T>), dim3((unsigned)(L < 65536 ? L : 65536)), dim3(64), 0, s, L, nm, n, a, q,
+                     (long long)nm, (long long)nm * n);
+  return hipGetLastError();
+}
+
+// ---- interleaved layout ---------------------------------------------------------
+
+size_t interleaved_bytes(int L, int nm, int n, int m, int tb, int solver)
+{
+  if (bad_args(L, nm, n, m) || (tb != 4 && tb != 8)) return 0;
+  const size_t inner = workspace_bytes(L, nm, n, m, tb, solver);
+  if (inner == 0) return 0;
+  return al256((size_t)L * nm * n * tb) + al256((size_t)L * n * tb) + inner;
+}
+
+hipError_t transpose_launch(long long R, long long C, const void *in, void *out, int tb, hipStream_t s)
+{
+  const long long tiles = ((R + 63) / 64) * ((C + 63) / 64);
+  const unsigned grid = (unsigned)(tiles < 65536 ? tiles : 65536);
+  if (tb == 8)
+    hipLaunchKernelGGL((eig_transpose_kernel<double>), dim3(grid), dim3(256), 0, s, R, C, (const double *)in,
+                       (double *)out);
+  else
+    hipLaunchKernelGGL((eig_transpose_kernel<float>), dim3(grid), dim3(256), 0, s, R, C, (const float *)in,
+                       (float *)out);
+  return hipGetLastError();
+}
+
+// Interleaved batch (element (r, c) of matrix k at a[k + L (r + c nm)],
+// eigenvalue j at w[k + L j]): transposed into the batch-strided layout in the
+// workspace, solved there, transposed back.
+template <typename T>
+hipError_t run_interleaved(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s, int solver, int *info)
+{
+  if (bad_args(L, nm, n, m) || !a || !w || !wk) return hipErrorInvalidValue;
+  const size_t inner = workspace_bytes(L, nm, n, m, (int)sizeof(T), solver);
+  if (inner == 0) return hipErrorNotSupported;
+  char *base = (char *)wk;
+  T *as = (T *)base;
+  T *ws = (T *)(base + al256((size_t)L * nm * n * sizeof(T)));
+  void *wk2 = base + al256((size_t)L * nm * n * sizeof(T)) + al256((size_t)L * n * sizeof(T));
+  const long long e = (long long)nm * n;
+  hipError_t err = transpose_launch(e, L, a, as, (int)sizeof(T), s);
+  if (err == hipSuccess) err = run<T>(L, nm, n, m, as, ws, wk2, s, solver, info);
+  if (err == hipSuccess) err = transpose_launch(L, e, as, a, (int)sizeof(T), s);
+  if (err == hipSuccess) err = transpose_launch(L, n, ws, w, (int)sizeof(T), s);
+  return err;
 }
 
 // ---- accuracy check -----------------------------------------------------------
@@ -1125,7 +1549,79 @@ int eigen_GPU_batch_launch_params(int L, int n, int dtype_bytes, int solver, int
   return (int)launch_params(L, n, dtype_bytes, solver, params);
 }
 
-const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.2"; }
+size_t eigen_GPU_batch_refine_bufsize(int L, int n)
+{
+  if (L < 1 || n < 1 || n > 256) return 0;
+  return WS_HEAD + RefineLayout(L, n).total;
+}
+
+int eigen_GPU_batch_refine_DP(int L, int nm, int n, const double *a, double *x, int ldx, double *w, void *wk,
+                              hipStream_t stream, int iters, int *info)
+{
+  return run_refine(L, nm, n, a, x, ldx, w, wk, stream, iters, info);
+}
+
+size_t eigen_GPU_batch_warm_bufsize(int L, int n, int dtype_bytes) { return warm_bytes(L, n, dtype_bytes); }
+
+int eigen_GPU_batch_warm_FP(int L, int nm, int n, int m, float *a, const float *q0, float *w, void *wk,
+                            hipStream_t stream, int max_sweeps, int *info, int *sweeps)
+{
+  return run_warm<float>(L, nm, n, m, a, q0, w, wk, stream, max_sweeps, info, sweeps);
+}
+
+int eigen_GPU_batch_warm_DP(int L, int nm, int n, int m, double *a, const double *q0, double *w, void *wk,
+                            hipStream_t stream, int max_sweeps, int *info, int *sweeps)
+{
+  return run_warm<double>(L, nm, n, m, a, q0, w, wk, stream, max_sweeps, info, sweeps);
+}
+
+int eigen_GPU_batch_qr_FP(int L, int nm, int n, const float *a, float *q, hipStream_t stream)
+{
+  return run_qr<float>(L, nm, n, a, q, stream);
+}
+
+int eigen_GPU_batch_qr_DP(int L, int nm, int n, const double *a, double *q, hipStream_t stream)
+{
+  return run_qr<double>(L, nm, n, a, q, stream);
+}
+
+size_t eigen_GPU_batch_interleaved_bufsize(int L, int nm, int n, int m, int dtype_bytes, int solver)
+{
+  return interleaved_bytes(L, nm, n, m, dtype_bytes, solver);
+}
+
+int eigen_GPU_batch_interleaved_FP(int L, int nm, int n, int m, float *a, float *w, void *wk, hipStream_t stream,
+                                   int solver, int *info)
+{
+  return run_interleaved<float>(L, nm, n, m, a, w, wk, stream, solver, info);
+}
+
+int eigen_GPU_batch_interleaved_DP(int L, int nm, int n, int m, double *a, double *w, void *wk,
+                                   hipStream_t stream, int solver, int *info)
+{
+  return run_interleaved<double>(L, nm, n, m, a, w, wk, stream, solver, info);
+}
+
+int eigen_GPU_batch_gemm(int kind, int transa, int transb, int M, int N, int K, double alpha, const void *A,
+                         long long lda, long long stride_a, const void *B, long long ldb, long long stride_b,
+                         double beta, const void *D, long long ldd, long long stride_d, void *C, long long ldc,
+                         long long stride_c, int batch, hipStream_t stream)
+{
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.batch = batch;
+  g.transa = transa ? 1 : 0;
+  g.transb = transb ? 1 : 0;
+  g.A = A; g.lda = lda; g.sa = stride_a;
+  g.B = B; g.ldb = ldb; g.sb = stride_b;
+  g.D = D; g.ldd = D ? ldd : 0; g.sd = D ? stride_d : 0;
+  g.C = C; g.ldc = ldc; g.sc = stride_c;
+  g.alpha = alpha;
+  g.beta = D ? beta : 0.0;
+  g.sym = 0;
+  return gemm_launch(kind, g, stream);
+}
+
+const char *eigen_GPU_batch_version(void) { return "eigen-mi355x 0.3"; }
 
 } // extern "C"
 

@@ -20,11 +20,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeigenGbatch.so")
 
 IMTQL2, TQL2, JACOBI = 0, 1, 2
+# FP64 interface only: FP32 solve + FP64 refinement on the matrix cores, with the
+# plain FP64 solver as fallback; bits 28-29 set the refinement iterations (0: 3)
+MIXED = 3
+MIX_ITERS_SHIFT = 28
+# batched products (eigen_GPU_batch_gemm): matrix-core operand kinds
+GEMM_F64, GEMM_F32, GEMM_F16, GEMM_BF16 = 0, 1, 2, 3
 # tuning flags OR-ed into the solver id of the _x entry points
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
 FLAG_QL_LDS = 0x800
 FLAG_TRI_FULL = 0x4000000
 FLAG_REG128 = 0x8000
+# eigenpairs in the QL / Jacobi order instead of ascending (the reference's DO_SORT 0)
+FLAG_UNSORTED = 0x40000000
 PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG,
          "split-lds": FLAG_SPLIT | FLAG_QL_LDS, "split-full": FLAG_SPLIT | FLAG_TRI_FULL,
          "split-reg128": FLAG_REG128, "split-reg128-fallback": FLAG_REG128 | FLAG_TINYLOG}
@@ -96,6 +104,30 @@ def lib() -> ctypes.CDLL:
         L.eigen_GPU_batch_bufsize.argtypes = [ci, ci, ci, ci, ci, ci]
         L.eigen_GPU_batch_bufsize.restype = sz
         L.eigen_GPU_batch_version.restype = ctypes.c_char_p
+        ll = ctypes.c_longlong
+        L.eigen_GPU_batch_refine_bufsize.argtypes = [ci, ci]
+        L.eigen_GPU_batch_refine_bufsize.restype = sz
+        L.eigen_GPU_batch_refine_DP.argtypes = [ci, ci, ci, vp, vp, ci, vp, vp, vp, ci, vp]
+        L.eigen_GPU_batch_refine_DP.restype = ci
+        L.eigen_GPU_batch_warm_bufsize.argtypes = [ci, ci, ci]
+        L.eigen_GPU_batch_warm_bufsize.restype = sz
+        for name in ("eigen_GPU_batch_warm_FP", "eigen_GPU_batch_warm_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, vp, vp]
+            f.restype = ci
+        for name in ("eigen_GPU_batch_qr_FP", "eigen_GPU_batch_qr_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, ci, vp, vp, vp]
+            f.restype = ci
+        L.eigen_GPU_batch_interleaved_bufsize.argtypes = [ci, ci, ci, ci, ci, ci]
+        L.eigen_GPU_batch_interleaved_bufsize.restype = sz
+        for name in ("eigen_GPU_batch_interleaved_FP", "eigen_GPU_batch_interleaved_DP"):
+            f = getattr(L, name)
+            f.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, ci, vp]
+            f.restype = ci
+        L.eigen_GPU_batch_gemm.argtypes = [ci, ci, ci, ci, ci, ci, cd, vp, ll, ll, vp, ll, ll, cd, vp, ll, ll,
+                                           vp, ll, ll, ci, vp]
+        L.eigen_GPU_batch_gemm.restype = ci
         _lib = L
     return _lib
 
@@ -119,7 +151,8 @@ def _stream_ptr(stream) -> int:
 
 
 def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None,
-                workspace=None, stream=None, abi: bool = False, path: str = "auto", phases=None):
+                workspace=None, stream=None, abi: bool = False, path: str = "auto", phases=None,
+                unsorted: bool = False):
     """Solve in place: ``a`` (L, n, nm) float32/float64 CUDA tensor.
 
     Returns (a, w, info): eigenvectors overwrite ``a`` (column j of matrix k is
@@ -133,6 +166,8 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     registers) or, for 64 < n <= 256, "split-full" (split with the
     tridiagonalisation on the full matrix instead of its lower triangle).
     phases (split path only): iterable of "tri", "ql", "apply" to run.
+    m < n: all n eigenvalues, eigenvectors of the first m only (columns m..n-1
+    of each matrix are left unspecified).  unsorted: the QL / Jacobi order.
     """
     import torch
     if not a.is_cuda:
@@ -142,7 +177,7 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     L, n, nm = a.shape
     m = n if m is None else m
     tb = a.element_size()
-    solver = solver | PATHS[path]
+    solver = solver | PATHS[path] | (FLAG_UNSORTED if unsorted else 0)
     if phases:
         for ph in phases:
             solver |= PHASES[ph]
@@ -163,6 +198,92 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
            info.data_ptr())
     if rc != 0:
         raise RuntimeError(f"eigen_GPU_batch_{'FP' if tb == 4 else 'DP'}_x failed: hipError {rc}")
+    return a, w, info
+
+
+def refine(a, x, iters: int = 3, w=None, info=None, workspace=None, stream=None):
+    """FP64 refinement of approximate eigenvectors (eigen_GPU_batch_refine_DP):
+    a (L, n, nm) float64 matrices (upper triangle read), x (L, n, ldx) float64
+    approximate eigenvectors (column j of matrix k = x[k, j, :n]), refined in
+    place.  Returns (x, w, info): info 0 accepted, else reason bits."""
+    import torch
+    L, n, nm = a.shape
+    ldx = x.shape[2]
+    if a.dtype != torch.float64 or x.dtype != torch.float64 or x.shape[:2] != (L, n):
+        raise ValueError("float64 tensors a (L, n, nm) and x (L, n, ldx)")
+    if w is None:
+        w = torch.empty((L, n), dtype=torch.float64, device=a.device)
+    if info is None:
+        info = torch.empty(L, dtype=torch.int32, device=a.device)
+    if workspace is None:
+        workspace = torch.empty(max(int(lib().eigen_GPU_batch_refine_bufsize(L, n)), 1), dtype=torch.uint8,
+                                device=a.device)
+    _check_rc(lib().eigen_GPU_batch_refine_DP(L, nm, n, a.data_ptr(), x.data_ptr(), ldx, w.data_ptr(),
+                                              workspace.data_ptr(), _stream_ptr(stream), iters, info.data_ptr()),
+              "eigen_GPU_batch_refine_DP")
+    return x, w, info
+
+
+def qr(a, q=None, stream=None):
+    """Q factors of the Householder QR of a (L, n, nm) batch (eigen_GPU_batch_qr_*),
+    R with a non-negative diagonal; n <= 64.  Returns q (same layout)."""
+    import torch
+    L, n, nm = a.shape
+    if q is None:
+        q = torch.empty_like(a)
+    f = getattr(lib(), "eigen_GPU_batch_qr_" + _sfx(a))
+    _check_rc(f(L, nm, n, a.data_ptr(), q.data_ptr(), _stream_ptr(stream)), "eigen_GPU_batch_qr")
+    return q
+
+
+def eigen_batch_warm(a, q0, max_sweeps: int = 6, m: int | None = None, w=None, info=None, sweeps=None,
+                     workspace=None, stream=None):
+    """Warm-started solve (eigen_GPU_batch_warm_*): a (L, n, nm) solved in place
+    from the approximate eigenbases q0 (same layout), with the plain solver as
+    fallback past max_sweeps Jacobi sweeps.  Returns (a, w, info, sweeps)."""
+    import torch
+    L, n, nm = a.shape
+    m = n if m is None else m
+    if q0.shape != a.shape or q0.dtype != a.dtype:
+        raise ValueError("q0 must have a's shape and dtype")
+    if w is None:
+        w = torch.empty((L, n), dtype=a.dtype, device=a.device)
+    if info is None:
+        info = torch.empty(L, dtype=torch.int32, device=a.device)
+    if sweeps is None:
+        sweeps = torch.empty(L, dtype=torch.int32, device=a.device)
+    if workspace is None:
+        workspace = torch.empty(max(int(lib().eigen_GPU_batch_warm_bufsize(L, n, a.element_size())), 1),
+                                dtype=torch.uint8, device=a.device)
+    f = getattr(lib(), "eigen_GPU_batch_warm_" + _sfx(a))
+    _check_rc(f(L, nm, n, m, a.data_ptr(), q0.data_ptr(), w.data_ptr(), workspace.data_ptr(), _stream_ptr(stream),
+                max_sweeps, info.data_ptr(), sweeps.data_ptr()), "eigen_GPU_batch_warm")
+    return a, w, info, sweeps
+
+
+def eigen_batch_interleaved(a, L: int, nm: int, n: int, m: int | None = None, w=None, solver: int = IMTQL2,
+                            info=None, workspace=None, stream=None):
+    """Interleaved layout (eigen_GPU_batch_interleaved_*): ``a`` a contiguous
+    tensor of L*nm*n elements with element (r, c) of matrix k at
+    a[k + L*(r + c*nm)] (e.g. shape (n, nm, L)); eigenvalue j of matrix k goes
+    to w[k + L*j].  Solved in place; returns (a, w, info)."""
+    import torch
+    m = n if m is None else m
+    if not a.is_cuda or a.dtype not in (torch.float32, torch.float64) or not a.is_contiguous():
+        raise ValueError("a must be a contiguous float32/float64 device tensor")
+    if a.numel() != L * nm * n:
+        raise ValueError("a must hold L*nm*n elements")
+    tb = a.element_size()
+    if w is None:
+        w = torch.empty((n, L), dtype=a.dtype, device=a.device)
+    if info is None:
+        info = torch.empty(L, dtype=torch.int32, device=a.device)
+    if workspace is None:
+        workspace = torch.empty(max(int(lib().eigen_GPU_batch_interleaved_bufsize(L, nm, n, m, tb, solver)), 1),
+                                dtype=torch.uint8, device=a.device)
+    f = lib().eigen_GPU_batch_interleaved_FP if tb == 4 else lib().eigen_GPU_batch_interleaved_DP
+    _check_rc(f(L, nm, n, m, a.data_ptr(), w.data_ptr(), workspace.data_ptr(), _stream_ptr(stream), solver,
+                info.data_ptr()), "eigen_GPU_batch_interleaved")
     return a, w, info
 
 
@@ -357,6 +478,45 @@ def launch_params(L: int, n: int, dtype_bytes: int, solver: int = IMTQL2) -> dic
     out = dict(zip(keys, list(p)))
     out["rc"] = rc
     return out
+
+
+def gemm_raw(kind: int, transa: int, transb: int, M: int, N: int, K: int, alpha: float,
+             A, lda: int, sa: int, B, ldb: int, sb: int, beta: float, D, ldd: int, sd: int,
+             C, ldc: int, sc: int, batch: int, stream=None) -> int:
+    """eigen_GPU_batch_gemm on device tensors (their data pointers): C_b = alpha
+    op(A_b) op(B_b) + beta D_b, column-major, strides in elements.  Returns the
+    hipError code."""
+    return int(lib().eigen_GPU_batch_gemm(kind, transa, transb, M, N, K, alpha, A.data_ptr(), lda, sa,
+                                          B.data_ptr(), ldb, sb, beta,
+                                          None if D is None else D.data_ptr(), ldd, sd,
+                                          C.data_ptr(), ldc, sc, batch, _stream_ptr(stream)))
+
+
+def gemm(a, b, transa: bool = False, transb: bool = False, kind: int | None = None, alpha: float = 1.0,
+         beta: float = 0.0, d=None, c=None, stream=None):
+    """Batched product in the solver's layout: a (batch, cols, ld) holds the
+    column-major matrices A_b (element (r, c) at a[b, c, r]); likewise b, d, c.
+    Returns c with C_b = alpha op(A_b) op(B_b) + beta D_b (shape (batch, N, M)).
+    kind defaults to GEMM_F64 for float64 and GEMM_F32 for float32."""
+    import torch
+    if a.dtype != b.dtype or a.dim() != 3 or b.dim() != 3 or not (a.is_contiguous() and b.is_contiguous()):
+        raise ValueError("contiguous (batch, cols, ld) tensors of one dtype")
+    if kind is None:
+        kind = GEMM_F64 if a.dtype == torch.float64 else GEMM_F32
+    batch = a.shape[0]
+    ar, ac = a.shape[2], a.shape[1]
+    br, bc = b.shape[2], b.shape[1]
+    M, K = (ac, ar) if transa else (ar, ac)
+    K2, N = (bc, br) if transb else (br, bc)
+    if K != K2 or b.shape[0] != batch:
+        raise ValueError("inner dimensions or batch sizes differ")
+    if c is None:
+        c = torch.empty((batch, N, M), dtype=a.dtype, device=a.device)
+    rc = gemm_raw(kind, int(transa), int(transb), M, N, K, alpha, a, ar, ar * ac, b, br, br * bc, beta,
+                  d, M if d is None else d.shape[2], 0 if d is None else d.shape[1] * d.shape[2],
+                  c, c.shape[2], c.shape[1] * c.shape[2], batch, stream)
+    _check_rc(rc, "eigen_GPU_batch_gemm")
+    return c
 
 
 def tune_reset() -> None:

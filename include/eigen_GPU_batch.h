@@ -10,7 +10,11 @@
  *        the unit eigenvector of w[k*n + j].
  *   w  : L*n eigenvalues, ascending within each matrix, matrix k at w + k*n.
  *   wk : device workspace of at least lwork bytes (see the query below).
- *   m  : 1 <= m <= n.  All n eigenpairs are computed and returned for any m.
+ *   m  : 1 <= m <= n.  All n eigenvalues are returned in w; the eigenvectors
+ *        of the first m of them are written to columns 0..m-1 of a, and columns
+ *        m..n-1 of a are left unspecified (the reference passes m through but
+ *        stores inconsistently for m < n: w offset pos*m, eigen_GPU_batch.cu:89,
+ *        against stride n, :133; back-transform of m columns, hhtr2sy.hpp:116,144).
  *
  * Workspace query (eigen_GPU_batch.cu:329-359): calling with wk == NULL does not
  * compute; it writes the required workspace size in bytes as a size_t into the
@@ -52,8 +56,20 @@ void eigen_GPU_batch_FP(const int L, const int nm, const int n, const int m,
 enum eigen_GPU_batch_solver {
   EIGEN_GPU_BATCH_IMTQL2 = 0, /* Householder + implicit-shift QL + back-transform (default) */
   EIGEN_GPU_BATCH_TQL2 = 1,   /* Householder + explicit-shift QL (the reference's shipped QL) */
-  EIGEN_GPU_BATCH_JACOBI = 2  /* two-sided Jacobi on the full matrix (jac_tiled.hpp) */
+  EIGEN_GPU_BATCH_JACOBI = 2, /* two-sided Jacobi on the full matrix (jac_tiled.hpp) */
+  /* FP64 entry points only, n <= 256: mixed precision.  The matrices rounded to
+   * FP32 are solved by the FP32 path, then the FP32 eigenpairs are refined in
+   * FP64 on the matrix cores (Ogita-Aishima iteration; clusters of up to 8
+   * eigenvalues closer than the iteration's separation bound are resolved by
+   * Rayleigh-Ritz); a matrix the refinement does not accept (larger cluster,
+   * measures above 1e-6 before the last update, failed FP32 solve) is solved
+   * by the plain FP64 workgroup solver.  The reference lists "Half precision
+   * version + Iterative refinment" as future work (README:169). */
+  EIGEN_GPU_BATCH_MIXED = 3
 };
+/* bits 28-29 of `solver` with EIGEN_GPU_BATCH_MIXED: refinement iterations
+ * (1..3; 0 selects the default, 3). */
+#define EIGEN_GPU_BATCH_MIXED_ITERS_SHIFT 28
 
 /*
  * Tuning bits OR-ed into `solver` of the _x entry points and of
@@ -73,7 +89,11 @@ enum eigen_GPU_batch_solver {
  *   bits 16-19 / 20-23  workgroup size / resident workgroup count of the
  *          workgroup kernels; bits 24-25 tridiagonalisation shape of the
  *          n > 64 split path; 0x4000000 (n <= 256) its full-storage form
- *          instead of the default lower-triangle form.
+ *          instead of the default lower-triangle form;
+ *   0x40000000 eigenpairs in the order the QL iteration (or Jacobi) leaves
+ *          them instead of ascending: the reference's DO_SORT 0
+ *          (eigen_GPU_batch.cu:5,10; its README:57-58 calls unsorted the
+ *          default).  Columns 0..m-1 then hold the first m of that order.
  */
 #define EIGEN_GPU_BATCH_FUSED 0x100
 #define EIGEN_GPU_BATCH_SPLIT 0x200
@@ -83,6 +103,7 @@ enum eigen_GPU_batch_solver {
 #define EIGEN_GPU_BATCH_PHASE_APPLY 0x4000
 #define EIGEN_GPU_BATCH_REG128 0x8000
 #define EIGEN_GPU_BATCH_TRI_FULL 0x4000000
+#define EIGEN_GPU_BATCH_UNSORTED 0x40000000
 
 /* Workspace size in bytes for (L, nm, n, m, solver); 0 for invalid arguments. */
 size_t eigen_GPU_batch_bufsize(int L, int nm, int n, int m, int dtype_bytes, int solver);
@@ -260,6 +281,82 @@ int eigen_GPU_batch_FP_noalloc(int L, int nm, int n, int m, float *a, float *w, 
                                int solver, int *info);
 int eigen_GPU_batch_DP_noalloc(int L, int nm, int n, int m, double *a, double *w, hipStream_t stream,
                                int solver, int *info);
+
+/*
+ * FP64 refinement of approximate eigenpairs (no reference symbol; the second
+ * half of EIGEN_GPU_BATCH_MIXED, usable on eigenvectors from any source):
+ * a holds the L matrices (upper triangle and diagonal read, ld nm), x their
+ * approximate eigenvectors (n columns, ld ldx >= n, matrix k at x + k ldx n),
+ * refined in place by `iters` (1..8) Ogita-Aishima iterations on the matrix
+ * cores with Rayleigh-Ritz on clusters of up to 8 close eigenvalues; w gets
+ * the eigenvalues, ascending, with the columns of x in the same order.
+ * info[k] = 0 when the last update's first-order entries are at most 1e-6
+ * (the result is then at FP64 rounding level), else a positive bit set:
+ * 2 cluster beyond 8, 4 not converged, 8 non-finite values; x and w of such
+ * a matrix are left as the iteration reached them.  n <= 256; workspace of
+ * eigen_GPU_batch_refine_bufsize(L, n) bytes.  Returns hipSuccess,
+ * hipErrorInvalidValue, hipErrorNotSupported or the launch error.
+ */
+size_t eigen_GPU_batch_refine_bufsize(int L, int n);
+int eigen_GPU_batch_refine_DP(int L, int nm, int n, const double *a, double *x, int ldx, double *w, void *wk,
+                              hipStream_t stream, int iters, int *info);
+
+/*
+ * Warm-started solve, n <= 64 (no reference symbol): q0 holds an approximate
+ * eigenbasis per matrix (a's layout), e.g. the warm-start network's output.
+ * Q = qr(q0) (Householder, R with a non-negative diagonal), B = Q^T A Q and
+ * V = Q Z_B on the matrix cores (FP32 / FP64 operands), Z_B from cyclic
+ * Jacobi on B (the reference's jac_tiled, jac_tiled.hpp:3-152, started from
+ * Q instead of the identity) with at most max_sweeps (1..40) sweeps.  A
+ * matrix whose Jacobi has not converged within max_sweeps is solved by the
+ * plain solver (tridiagonalisation + QL) from its input.  Results as for the
+ * _x entry points (sorted, first m eigenvector columns); sweeps (device int[L]
+ * or NULL) gets the sweeps each Jacobi took (max_sweeps: fell back).  Workspace:
+ * eigen_GPU_batch_warm_bufsize(L, n, dtype_bytes).
+ */
+size_t eigen_GPU_batch_warm_bufsize(int L, int n, int dtype_bytes);
+int eigen_GPU_batch_warm_FP(int L, int nm, int n, int m, float *a, const float *q0, float *w, void *wk,
+                            hipStream_t stream, int max_sweeps, int *info, int *sweeps);
+int eigen_GPU_batch_warm_DP(int L, int nm, int n, int m, double *a, const double *q0, double *w, void *wk,
+                            hipStream_t stream, int max_sweeps, int *info, int *sweeps);
+/* Q factor of the Householder QR of each of L matrices (n <= 64, ld nm, R with
+ * a non-negative diagonal); q may equal a.  L == 0 is a no-op. */
+int eigen_GPU_batch_qr_FP(int L, int nm, int n, const float *a, float *q, hipStream_t stream);
+int eigen_GPU_batch_qr_DP(int L, int nm, int n, const double *a, double *q, hipStream_t stream);
+
+/*
+ * Interleaved batch layout (the reference's to-do "flexible interleave data
+ * format", README:166; no reference symbol): element (r, c) of matrix k at
+ * a[k + L (r + c nm)] and eigenvalue j of matrix k at w[k + L j], i.e. the
+ * matrix index runs fastest.  Same arguments, options and results as the _x
+ * entry points; the batch is transposed into the batch-strided layout inside
+ * the workspace (eigen_GPU_batch_interleaved_bufsize bytes: the strided copy,
+ * its eigenvalues and the _x workspace), solved, and transposed back.
+ */
+size_t eigen_GPU_batch_interleaved_bufsize(int L, int nm, int n, int m, int dtype_bytes, int solver);
+int eigen_GPU_batch_interleaved_FP(int L, int nm, int n, int m, float *a, float *w, void *wk,
+                                   hipStream_t stream, int solver, int *info);
+int eigen_GPU_batch_interleaved_DP(int L, int nm, int n, int m, double *a, double *w, void *wk,
+                                   hipStream_t stream, int solver, int *info);
+
+/*
+ * Batched products on the matrix cores (no reference symbol; the reference
+ * reduces its back-transformation products with shuffles, hhtr2sy.hpp:144-202):
+ *   C_b = alpha op(A_b) op(B_b) + beta D_b,  b = 0 .. batch-1,
+ * column-major, op(X) = X (trans 0) or X^T (trans 1), op(A_b) M x K, op(B_b)
+ * K x N, leading dimensions and batch strides in elements.  kind: 0 FP64
+ * operands (v_mfma_f64_16x16x4f64), 1 FP32 (v_mfma_f32_16x16x4f32), 2 FP32
+ * storage multiplied as FP16 (v_mfma_f32_16x16x16f16), 3 FP32 storage
+ * multiplied as bfloat16 (v_mfma_f32_16x16x16bf16_1k); FP32 accumulation for
+ * kinds 1-3.  D may be NULL (beta ignored) or equal to C (same ldc); C must not
+ * overlap A or B.  Returns hipSuccess (batch == 0: nothing launched),
+ * hipErrorInvalidValue, or the launch error.
+ */
+int eigen_GPU_batch_gemm(int kind, int transa, int transb, int M, int N, int K, double alpha,
+                         const void *A, long long lda, long long stride_a,
+                         const void *B, long long ldb, long long stride_b, double beta,
+                         const void *D, long long ldd, long long stride_d,
+                         void *C, long long ldc, long long stride_c, int batch, hipStream_t stream);
 
 /* Library version string ("eigen-mi355x <major>.<minor>"). */
 const char *eigen_GPU_batch_version(void);

@@ -204,3 +204,42 @@ def test_uneven_workspace_sizes():
     assert 65536 * per <= big <= 65536 * per + small + (256 * 128 * 128 * 4) + (64 << 10)
     assert eg.uneven_bufsize(1000, 128, 4) < big                 # chunks never exceed the batch
     assert eg.uneven_bufsize(100000, 200, 8) > eg.uneven_bufsize(100000, 64, 8)
+
+
+def test_gemm_validates_arguments_without_a_gpu():
+    """eigen_GPU_batch_gemm rejects bad shapes, leading dimensions, kinds and
+    overlapping outputs before any HIP call (hipErrorInvalidValue = 1);
+    batch == 0 is a successful no-op."""
+    f = eg.lib().eigen_GPU_batch_gemm
+    A, B, C = 1 << 20, 2 << 20, 3 << 20               # never dereferenced
+    ok = dict(kind=0, ta=0, tb=0, M=8, N=8, K=8, A=A, lda=8, sa=64, B=B, ldb=8, sb=64, C=C, ldc=8, sc=64, batch=0)
+
+    def call(**kw):
+        a = dict(ok, **kw)
+        return f(a["kind"], a["ta"], a["tb"], a["M"], a["N"], a["K"], 1.0, a["A"], a["lda"], a["sa"],
+                 a["B"], a["ldb"], a["sb"], 0.0, None, 0, 0, a["C"], a["ldc"], a["sc"], a["batch"], None)
+    assert call() == 0
+    for bad in (dict(kind=4), dict(kind=-1), dict(M=0), dict(K=0), dict(batch=-1), dict(lda=7, batch=1),
+                dict(ta=1, lda=7, batch=1), dict(ldb=7, batch=1), dict(ldc=7, batch=1),
+                dict(C=A + 8, batch=1), dict(B=C, batch=1), dict(sa=-1, batch=1)):
+        assert call(**bad) == 1, bad
+
+
+def test_mixed_workspace_sizes():
+    """The mixed-precision solver: FP64 interface and n <= 256 only; its
+    workspace holds the FP32 copy, the FP32 solver's workspace and one
+    refinement chunk (at most 2048 matrices) of six n x n FP64 arrays."""
+    assert eg.lib().eigen_GPU_batch_refine_bufsize(10, 257) == 0
+    assert eg.lib().eigen_GPU_batch_refine_bufsize(0, 16) == 0
+    assert eg.lib().eigen_GPU_batch_warm_bufsize(10, 65, 4) == 0
+    assert eg.lib().eigen_GPU_batch_warm_bufsize(100, 48, 4) >= 3 * 100 * 48 * 48 * 4
+    assert eg.bufsize(100, 64, 64, 64, 4, eg.MIXED) == 0
+    assert eg.bufsize(100, 257, 257, 257, 8, eg.MIXED) == 0
+    n = 256
+    small = eg.bufsize(100, n, n, n, 8, eg.MIXED)
+    big = eg.bufsize(8192, n, n, n, 8, eg.MIXED)
+    assert small > 100 * 6 * n * n * 8
+    fp32 = eg.bufsize(8192, n, n, n, 4, eg.IMTQL2)
+    assert big >= fp32 + 8192 * n * n * 4 + 2048 * 6 * n * n * 8
+    # plus the eigenvalues / status of the FP32 solve and the refinement's per-matrix results
+    assert big <= fp32 + 8192 * n * n * 4 + 2048 * 6 * n * n * 8 + 8192 * (n + 1) * 8 + 2048 * (160 * n + 64) + (1 << 20)

@@ -231,3 +231,53 @@ def test_classical_jacobi_special_cases(dtype):
     _, w = solve(af, oracle.JACOBI_CLASSICAL)
     lam = wl.frank_eigenvalues(24)
     assert np.max(np.abs(w - lam) / lam[-1]) <= 64 * 24 * eps_of(dtype)
+
+
+def test_refinement_restatement_converges_to_lapack():
+    """oracle/refine_oa.py (the restatement the GPU refinement is checked
+    against) is pinned by what the mathematics fixes: from FP32 eigenvectors,
+    three steps reach FP64 eigenpairs of LAPACK's accuracy (eigenvalues equal to
+    numpy's eigvalsh within 1e-13 max|l|, residual and orthogonality ~1e-14),
+    and clusters below FP32 resolution (pairs split by 1e-8) come out resolved."""
+    import refine_oa
+    for n in (2, 7, 48, 120):
+        lam = wl.clustered_uniform(3, n, frac=0.1, gap=1e-8, seed=n)
+        full = wl.unpack(wl.qdq(lam, dtype=np.float64, seed=n + 1))
+        for k in range(3):
+            A = full[k]
+            _, v = np.linalg.eigh(A.astype(np.float32))
+            X = v.astype(np.float64)
+            ems = []
+            for _ in range(3):
+                X, l, em, big = refine_oa.refine_step(A, X)
+                ems.append(em)
+                assert not big
+            order = np.argsort(l)
+            z, w = X[:, order], l[order]
+            ref = np.linalg.eigvalsh(A)
+            assert np.max(np.abs(w - ref)) <= 1e-13 * np.abs(ref).max()
+            assert np.linalg.norm(A @ z - z * w) <= 1e-13 * np.abs(w).max()
+            assert np.linalg.norm(z.T @ z - np.eye(n)) <= 1e-13
+            assert ems[2] <= 1e-6          # the GPU's acceptance bound on the last step
+
+
+def test_refinement_restatement_catches_a_planted_mistake(monkeypatch):
+    """A sign error in the first-order formula (E_ij with l_i - l_j) no longer
+    converges: the check above would fail."""
+    import refine_oa
+    n = 30
+    full = wl.unpack(wl.qdq(wl.clustered_uniform(1, n, seed=4), dtype=np.float64, seed=5))
+    A = full[0]
+    _, v = np.linalg.eigh(A.astype(np.float32))
+    X = v.astype(np.float64)
+    orig = refine_oa.refine_step
+
+    def bad_step(A, X):
+        Xn, l, em, big = orig(A, X)
+        # the same step with E's off-diagonal first-order part negated
+        E = np.linalg.lstsq(X, Xn, rcond=None)[0] - np.eye(len(l))
+        D = np.diag(np.diag(E))
+        return X + X @ (D - (E - D)), l, em, big
+    for _ in range(3):
+        X, l, _, _ = bad_step(A, X)
+    assert np.linalg.norm(A @ X - X * l) > 1e-10
