@@ -585,6 +585,108 @@ def bench_mixed(q0, plain_ms, steps, warmup):
             "gemm_roofline": gemm_f64_roofline(2048, n)}
 
 
+def timed(fn, reps=3):
+    """Mean HIP-event time (ms) of fn() on the current stream (one untimed call first)."""
+    import numpy as np
+    import torch
+    fn()
+    s = torch.cuda.current_stream()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.mean(ts))
+
+
+def bench_warmstart(L=65536, n=48, p=300, rho=1.05, hidden=512, train_steps=50, total_steps=200,
+                    eval_steps=(50, 100, 150, 199), max_sweeps=6, batch=4096, lr=1e-3):
+    """The learned warm start on the drifting stream: Y_t = Y_{t-1} + N(0, 0.01^2),
+    Y_0 ~ N(0, 0.1^2), L matrices A_t = ((k-1)/rho) I + Y_t^T Y_t (k = n = 48,
+    p = 300, rho = 1.05).  The network (3 layers, hidden 512) is trained by the
+    library's kernels on the plain solver's eigenpairs of steps 0..train_steps-1
+    (one epoch per step, minibatches of 4096, Adam); at the evaluation steps the
+    plain solve, the network + warm solve, and the ablations (identity start,
+    previous step's eigenvectors) are timed and checked."""
+    import numpy as np
+    import torch
+    import eigeng_batched as eg
+    import warmstart
+    torch.manual_seed(0)
+    stream = warmstart.DriftingStream(L, n, p=p, rho=rho)
+    net = warmstart.WarmNet(n, hidden=hidden, batch=batch, seed=1)
+    t0 = time.perf_counter()
+    hist = warmstart.train_on_stream(net, stream, steps=train_steps, epochs=1, lr=lr)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    ws_warm = torch.empty(int(eg.lib().eigen_GPU_batch_warm_bufsize(L, n, 4)), dtype=torch.uint8, device="cuda")
+    ws_plain = torch.empty(max(eg.bufsize(L, n, n, n, 4), 1), dtype=torch.uint8, device="cuda")
+    eye = torch.eye(n, device="cuda").expand(L, n, n).contiguous()
+    evals = []
+    prev_v = None
+    while stream.t < total_steps:
+        t = stream.t
+        need_prev = (t + 1) in eval_steps
+        if t in eval_steps or need_prev:
+            a = stream.matrices()
+            z = torch.empty_like(a)
+            w = torch.empty((L, n), device="cuda")
+            info = torch.empty(L, dtype=torch.int32, device="cuda")
+
+            def plain():
+                z.copy_(a)
+                eg.eigen_batch(z, w=w, info=info, workspace=ws_plain)
+            if t in eval_steps:
+                ms_plain = timed(plain)
+                acc_plain = accuracy_all(a, z, w)
+                q0 = torch.empty_like(a)
+                sw = torch.empty(L, dtype=torch.int32, device="cuda")
+                res = {"step": t, "plain_ms": ms_plain, "plain_accuracy": acc_plain}
+                starts = {"network": None, "identity": eye}
+                if prev_v is not None:
+                    starts["previous_step"] = prev_v
+                for name, q in starts.items():
+                    def run(q=q):
+                        z.copy_(a)
+                        qq = net.forward(a, q0) if q is None else q
+                        eg.eigen_batch_warm(z, qq, max_sweeps=max_sweeps, w=w, info=info, sweeps=sw,
+                                            workspace=ws_warm)
+                    ms = timed(run)
+                    acc = accuracy_all(a, z, w)
+                    swn = sw.cpu().numpy()
+                    res[name] = {"ms": ms, "matrices_per_s": L / (ms * 1e-3), "vs_plain": ms_plain / ms,
+                                 "mean_sweeps": float(swn.mean()),
+                                 "fallbacks": int((swn >= max_sweeps).sum()), "accuracy_all": acc}
+                res["network_forward_ms"] = timed(lambda: net.forward(a, q0))
+                evals.append(res)
+                print(f"[warmstart] step {t}: plain {ms_plain:.2f} ms, network {res['network']['ms']:.2f} ms "
+                      f"({res['network']['mean_sweeps']:.2f} sweeps), identity {res['identity']['ms']:.2f} ms "
+                      f"({res['identity']['mean_sweeps']:.2f} sweeps)", file=sys.stderr, flush=True)
+            if need_prev:
+                plain()
+                prev_v = z.clone()
+            del a, z
+        stream.advance()
+    del stream
+    torch.cuda.empty_cache()
+    mean = lambda k, f: float(np.mean([e[k][f] for e in evals if k in e]))
+    return {"workload": f"drifting stream of L={L} LETKF matrices n={n} FP32, k={n} p={p} rho={rho}, "
+                        f"Y_t = Y_(t-1) + N(0, 0.01^2), Y_0 ~ N(0, 0.1^2); network trained on steps "
+                        f"0..{train_steps - 1}, evaluated at steps {list(eval_steps)}",
+            "network": {"layers": [n * (n + 1) // 2, hidden, hidden, n * n], "params": net.count,
+                        "train_seconds": train_s, "adam_steps": net.step,
+                        "loss_first_last": [hist[0], hist[-1]]},
+            "max_sweeps": max_sweeps,
+            "value": L / (mean("network", "ms") * 1e-3), "unit": "matrices/s",
+            "plain_value": L / (float(np.mean([e["plain_ms"] for e in evals])) * 1e-3),
+            "vs_plain": {k: mean(k, "vs_plain") for k in ("network", "identity", "previous_step")},
+            "mean_sweeps": {k: mean(k, "mean_sweeps") for k in ("network", "identity", "previous_step")},
+            "evals": evals}
+
+
 def bench_n128(steps, warmup):
     """L = 8192, n = 128, FP32 line (the register split path, the default for
     FP32 64 < n <= 128 from 2048 matrices): whole-call time, per-phase times,
@@ -722,6 +824,8 @@ def main():
     ap.add_argument("--letkf-step-only", action="store_true", help="print only the LETKF analysis-step line")
     ap.add_argument("--n256-only", action="store_true",
                     help="run only the n=256 FP64 line (for profiling its kernels)")
+    ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start line")
+    ap.add_argument("--warmstart-only", action="store_true", help="print only the learned warm-start line")
     ap.add_argument("--mixed-only", action="store_true",
                     help="print only the n=256 FP64 mixed-precision line (and the plain time)")
     ap.add_argument("--uneven-count", type=int, default=100000)
@@ -737,6 +841,9 @@ def main():
     import torch
     import eigeng_batched as eg
 
+    if args.warmstart_only:
+        print(json.dumps({"warmstart": bench_warmstart()}))
+        return
     if args.mixed_only:
         import torch
         q0 = make_qdq_gpu(8192, 256, seed=0xE16E0004, dtype=torch.float64)
@@ -837,6 +944,8 @@ def main():
         out["letkf48"] = bench_letkf48(5, 1)
     if not args.no_letkf_step:
         out["letkf_step"] = bench_letkf_step(10, 2)
+    if not args.no_warmstart:
+        out["warmstart"] = bench_warmstart()
     if not args.no_uneven:
         del a, a0
         torch.cuda.empty_cache()

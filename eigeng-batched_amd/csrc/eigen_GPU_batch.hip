@@ -23,6 +23,7 @@
 #include "eig_refine.hpp"
 #include "eig_layout.hpp"
 #include "eig_warm.hpp"
+#include "eig_warmnet.hpp"
 
 #include <stdio.h>
 #include <vector>
@@ -1161,6 +1162,120 @@ hipError_t run_qr(int L, int nm, int n, const T *a, T *q, hipStream_t s)
   return hipGetLastError();
 }
 
+// ---- warm-start network (eig_warmnet.hpp) ------------------------------------------
+
+bool wn_ok(int n, int h) { return n >= 1 && n <= 64 && h >= 1 && h <= 4096; }
+
+// activations of a batch of B: features, the two hidden layers, the output (also
+// its gradient) and the two hidden gradients
+struct WnLayout {
+  size_t x, h1, h2, y, d2, d1, total;
+  WnLayout(int B, const WnDims &d)
+  {
+    x = 0;
+    h1 = x + al256((size_t)d.din * B * 4);
+    h2 = h1 + al256((size_t)d.h * B * 4);
+    y = h2 + al256((size_t)d.h * B * 4);
+    d2 = y + al256((size_t)d.dout * B * 4);
+    d1 = d2 + al256((size_t)d.h * B * 4);
+    total = d1 + al256((size_t)d.h * B * 4);
+  }
+};
+
+unsigned wn_grid(long long work) { return (unsigned)((work + 255) / 256 < 65536 ? (work + 255) / 256 : 65536); }
+
+// one product C = op(A) op(B) on the matrix cores (FP32 operands)
+hipError_t wn_gemm(int M, int N, int K, int ta, int tb, const float *A, int lda, const float *B, int ldb, float *C,
+                   int ldc, hipStream_t s)
+{
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.batch = 1;
+  g.transa = ta; g.transb = tb;
+  g.A = A; g.lda = lda;
+  g.B = B; g.ldb = ldb;
+  g.C = C; g.ldc = ldc;
+  g.alpha = 1.0;
+  return gemm_launch(GK_F32, g, s);
+}
+
+hipError_t wn_forward(const WnDims &d, int B, const float *p, char *ws, const WnLayout &ly, hipStream_t s)
+{
+  float *x = (float *)(ws + ly.x), *h1 = (float *)(ws + ly.h1), *h2 = (float *)(ws + ly.h2);
+  float *y = (float *)(ws + ly.y);
+  hipError_t err;
+  if ((err = wn_gemm(d.h, B, d.din, 1, 0, p + d.w1, d.din, x, d.din, h1, d.h, s)) != hipSuccess) return err;
+  hipLaunchKernelGGL(wn_bias_act_kernel, dim3(wn_grid((long long)d.h * B)), dim3(256), 0, s, d.h, (long long)B, h1,
+                     (long long)d.h, p + d.b1, 1);
+  if ((err = wn_gemm(d.h, B, d.h, 1, 0, p + d.w2, d.h, h1, d.h, h2, d.h, s)) != hipSuccess) return err;
+  hipLaunchKernelGGL(wn_bias_act_kernel, dim3(wn_grid((long long)d.h * B)), dim3(256), 0, s, d.h, (long long)B, h2,
+                     (long long)d.h, p + d.b2, 1);
+  if ((err = wn_gemm(d.dout, B, d.h, 1, 0, p + d.w3, d.h, h2, d.h, y, d.dout, s)) != hipSuccess) return err;
+  hipLaunchKernelGGL(wn_bias_act_kernel, dim3(wn_grid((long long)d.dout * B)), dim3(256), 0, s, d.dout,
+                     (long long)B, y, (long long)d.dout, p + d.b3, 0);
+  return hipGetLastError();
+}
+
+hipError_t wn_run_forward(int L, int nm, int n, int h, const float *p, const float *a, float *q0, int batch, void *wk,
+                          hipStream_t s)
+{
+  if (!wn_ok(n, h) || L < 0 || nm < n || batch < 1 || !p || !a || !q0 || !wk) return hipErrorInvalidValue;
+  const WnDims d = wn_dims(n, h);
+  const WnLayout ly(batch, d);
+  char *ws = (char *)wk;
+  for (long long c0 = 0; c0 < L; c0 += batch) {
+    const int cur = (int)((L - c0) < batch ? (L - c0) : batch);
+    hipLaunchKernelGGL(wn_features_kernel, dim3((unsigned)(cur < 65536 ? cur : 65536)), dim3(256), 0, s, cur, nm, n,
+                       a + (size_t)c0 * nm * n, (float *)(ws + ly.x));
+    hipError_t err = wn_forward(d, cur, p, ws, ly, s);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(wn_store_q_kernel, dim3(wn_grid((long long)cur * n * n)), dim3(256), 0, s, cur, nm, n,
+                       (const float *)(ws + ly.y), q0 + (size_t)c0 * nm * n);
+  }
+  return hipGetLastError();
+}
+
+void wn_rowsum(int rows, long long cols, const float *z, long long ld, float *db, hipStream_t s)
+{
+  const int split = cols >= 4096 ? 64 : (cols >= 256 ? 8 : 1);
+  hipLaunchKernelGGL(wn_rowsum_kernel, dim3((rows + 255) / 256, split), dim3(256), 0, s, rows, cols, z, ld, db, split);
+}
+
+hipError_t wn_run_grad(int B, int nm, int n, int h, const float *p, const float *a, const float *v, float *g,
+                       float *loss, void *wk, hipStream_t s)
+{
+  if (!wn_ok(n, h) || B < 1 || nm < n || !p || !a || !v || !g || !loss || !wk) return hipErrorInvalidValue;
+  const WnDims d = wn_dims(n, h);
+  const WnLayout ly(B, d);
+  char *ws = (char *)wk;
+  float *x = (float *)(ws + ly.x), *h1 = (float *)(ws + ly.h1), *h2 = (float *)(ws + ly.h2);
+  float *y = (float *)(ws + ly.y), *d2 = (float *)(ws + ly.d2), *d1 = (float *)(ws + ly.d1);
+  hipLaunchKernelGGL(wn_features_kernel, dim3((unsigned)(B < 65536 ? B : 65536)), dim3(256), 0, s, B, nm, n, a, x);
+  hipError_t err = wn_forward(d, B, p, ws, ly, s);
+  if (err != hipSuccess) return err;
+  // the bias gradients accumulate: clear them (and the loss) first
+  hipLaunchKernelGGL(wn_fill_kernel, dim3(1), dim3(256), 0, s, 1ll, loss, 0.0f);
+  hipLaunchKernelGGL(wn_fill_kernel, dim3(wn_grid(d.h)), dim3(256), 0, s, (long long)d.h, g + d.b1, 0.0f);
+  hipLaunchKernelGGL(wn_fill_kernel, dim3(wn_grid(d.h)), dim3(256), 0, s, (long long)d.h, g + d.b2, 0.0f);
+  hipLaunchKernelGGL(wn_fill_kernel, dim3(wn_grid(d.dout)), dim3(256), 0, s, (long long)d.dout, g + d.b3, 0.0f);
+  hipLaunchKernelGGL(wn_mse_grad_kernel, dim3(wn_grid((long long)B * n)), dim3(256), 0, s, B, nm, n, y, v, loss);
+  // layer 3
+  if ((err = wn_gemm(d.h, d.dout, B, 0, 1, h2, d.h, y, d.dout, g + d.w3, d.h, s)) != hipSuccess) return err;
+  wn_rowsum(d.dout, B, y, d.dout, g + d.b3, s);
+  if ((err = wn_gemm(d.h, B, d.dout, 0, 0, p + d.w3, d.h, y, d.dout, d2, d.h, s)) != hipSuccess) return err;
+  hipLaunchKernelGGL(wn_relu_grad_kernel, dim3(wn_grid((long long)d.h * B)), dim3(256), 0, s, (long long)d.h * B, d2,
+                     (const float *)h2);
+  // layer 2
+  if ((err = wn_gemm(d.h, d.h, B, 0, 1, h1, d.h, d2, d.h, g + d.w2, d.h, s)) != hipSuccess) return err;
+  wn_rowsum(d.h, B, d2, d.h, g + d.b2, s);
+  if ((err = wn_gemm(d.h, B, d.h, 0, 0, p + d.w2, d.h, d2, d.h, d1, d.h, s)) != hipSuccess) return err;
+  hipLaunchKernelGGL(wn_relu_grad_kernel, dim3(wn_grid((long long)d.h * B)), dim3(256), 0, s, (long long)d.h * B, d1,
+                     (const float *)h1);
+  // layer 1
+  if ((err = wn_gemm(d.din, d.h, B, 0, 1, x, d.din, d1, d.h, g + d.w1, d.din, s)) != hipSuccess) return err;
+  wn_rowsum(d.h, B, d1, d.h, g + d.b1, s);
+  return hipGetLastError();
+}
+
 // ---- interleaved layout ---------------------------------------------------------
 
 size_t interleaved_bytes(int L, int nm, int n, int m, int tb, int solver)
@@ -1547,6 +1662,48 @@ int eigen_GPU_batch_launch_params(int L, int n, int dtype_bytes, int solver, int
 {
   if (!params) return (int)hipErrorInvalidValue;
   return (int)launch_params(L, n, dtype_bytes, solver, params);
+}
+
+size_t eigen_GPU_warmnet_params(int n, int hidden)
+{
+  return wn_ok(n, hidden) ? (size_t)wn_dims(n, hidden).count : 0;
+}
+
+size_t eigen_GPU_warmnet_bufsize(int batch, int n, int hidden)
+{
+  if (!wn_ok(n, hidden) || batch < 1) return 0;
+  return WnLayout(batch, wn_dims(n, hidden)).total;
+}
+
+int eigen_GPU_warmnet_init(int n, int hidden, float *params, unsigned long long seed, hipStream_t stream)
+{
+  if (!wn_ok(n, hidden) || !params) return hipErrorInvalidValue;
+  const WnDims d = wn_dims(n, hidden);
+  hipLaunchKernelGGL(wn_init_kernel, dim3(wn_grid(d.count)), dim3(256), 0, stream, d, params, seed);
+  return hipGetLastError();
+}
+
+int eigen_GPU_warmnet_forward(int L, int nm, int n, int hidden, const float *params, const float *a, float *q0,
+                              int batch, void *wk, hipStream_t stream)
+{
+  return wn_run_forward(L, nm, n, hidden, params, a, q0, batch, wk, stream);
+}
+
+int eigen_GPU_warmnet_grad(int B, int nm, int n, int hidden, const float *params, const float *a, const float *v,
+                           float *grad, float *loss, void *wk, hipStream_t stream)
+{
+  return wn_run_grad(B, nm, n, hidden, params, a, v, grad, loss, wk, stream);
+}
+
+int eigen_GPU_warmnet_adam(long long count, float *params, const float *grad, float *m, float *v, int step, float lr,
+                           hipStream_t stream)
+{
+  if (count < 1 || step < 1 || !params || !grad || !m || !v || !(lr > 0.0f)) return hipErrorInvalidValue;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
+  hipLaunchKernelGGL(wn_adam_kernel, dim3(wn_grid(count)), dim3(256), 0, stream, count, params, grad, m, v, lr, b1, b2,
+                     1e-8f, c1, c2);
+  return hipGetLastError();
 }
 
 size_t eigen_GPU_batch_refine_bufsize(int L, int n)

@@ -105,6 +105,19 @@ def lib() -> ctypes.CDLL:
         L.eigen_GPU_batch_bufsize.restype = sz
         L.eigen_GPU_batch_version.restype = ctypes.c_char_p
         ll = ctypes.c_longlong
+        ull = ctypes.c_ulonglong
+        L.eigen_GPU_warmnet_params.argtypes = [ci, ci]
+        L.eigen_GPU_warmnet_params.restype = sz
+        L.eigen_GPU_warmnet_bufsize.argtypes = [ci, ci, ci]
+        L.eigen_GPU_warmnet_bufsize.restype = sz
+        L.eigen_GPU_warmnet_init.argtypes = [ci, ci, vp, ull, vp]
+        L.eigen_GPU_warmnet_init.restype = ci
+        L.eigen_GPU_warmnet_forward.argtypes = [ci, ci, ci, ci, vp, vp, vp, ci, vp, vp]
+        L.eigen_GPU_warmnet_forward.restype = ci
+        L.eigen_GPU_warmnet_grad.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp]
+        L.eigen_GPU_warmnet_grad.restype = ci
+        L.eigen_GPU_warmnet_adam.argtypes = [ll, vp, vp, vp, vp, ci, cf, vp]
+        L.eigen_GPU_warmnet_adam.restype = ci
         L.eigen_GPU_batch_refine_bufsize.argtypes = [ci, ci]
         L.eigen_GPU_batch_refine_bufsize.restype = sz
         L.eigen_GPU_batch_refine_DP.argtypes = [ci, ci, ci, vp, vp, ci, vp, vp, vp, ci, vp]

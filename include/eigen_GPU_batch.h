@@ -319,6 +319,35 @@ int eigen_GPU_batch_warm_FP(int L, int nm, int n, int m, float *a, const float *
                             hipStream_t stream, int max_sweeps, int *info, int *sweeps);
 int eigen_GPU_batch_warm_DP(int L, int nm, int n, int m, double *a, const double *q0, double *w, void *wk,
                             hipStream_t stream, int max_sweeps, int *info, int *sweeps);
+/*
+ * Warm-start network (no reference symbol): a 3-layer perceptron, hidden
+ * width `hidden` (ReLU), from the upper triangle of (A - mu I) / s (mu =
+ * trace / n, s the RMS of that triangle) to an n x n approximate eigenbasis
+ * Q0 for eigen_GPU_batch_warm_FP, n <= 64.  Parameters are one FP32 array of
+ * eigen_GPU_warmnet_params(n, hidden) floats: W1 (Din x H), b1, W2 (H x H), b2,
+ * W3 (H x n^2), b3 (column-major, Din = n(n+1)/2).  The layers' products run
+ * on the matrix cores (FP32 operands); wk holds the activations of `batch`
+ * matrices (eigen_GPU_warmnet_bufsize(batch, n, hidden) bytes).
+ *   init:    He-normal weights from `seed`, zero biases.
+ *   forward: q0 (a's layout, ld nm) for L matrices, processed `batch` at a time.
+ *   grad:    for B matrices a with target eigenvectors v (a's layout, columns
+ *            ascending), the mean squared error of the output against v with
+ *            each column's sign fixed (largest entry positive) into *loss
+ *            (device float) and its gradient into grad (the parameters' layout).
+ *   adam:    one Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected; step >= 1).
+ * All asynchronous on stream; return hipSuccess, hipErrorInvalidValue or the
+ * launch error.
+ */
+size_t eigen_GPU_warmnet_params(int n, int hidden);
+size_t eigen_GPU_warmnet_bufsize(int batch, int n, int hidden);
+int eigen_GPU_warmnet_init(int n, int hidden, float *params, unsigned long long seed, hipStream_t stream);
+int eigen_GPU_warmnet_forward(int L, int nm, int n, int hidden, const float *params, const float *a, float *q0,
+                              int batch, void *wk, hipStream_t stream);
+int eigen_GPU_warmnet_grad(int B, int nm, int n, int hidden, const float *params, const float *a, const float *v,
+                           float *grad, float *loss, void *wk, hipStream_t stream);
+int eigen_GPU_warmnet_adam(long long count, float *params, const float *grad, float *m, float *v, int step, float lr,
+                           hipStream_t stream);
+
 /* Q factor of the Householder QR of each of L matrices (n <= 64, ld nm, R with
  * a non-negative diagonal); q may equal a.  L == 0 is a no-op. */
 int eigen_GPU_batch_qr_FP(int L, int nm, int n, const float *a, float *q, hipStream_t stream);

@@ -29,7 +29,7 @@ def header_functions():
             continue
         txt = open(os.path.join(INC, f)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(eigen_GPU_batch\w*)\s*\(", txt, flags=re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(eigen_GPU_\w*)\s*\(", txt, flags=re.M):
             names.add(m.group(1))
     return sorted(names)
 

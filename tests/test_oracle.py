@@ -281,3 +281,33 @@ def test_refinement_restatement_catches_a_planted_mistake(monkeypatch):
     for _ in range(3):
         X, l, _, _ = bad_step(A, X)
     assert np.linalg.norm(A @ X - X * l) > 1e-10
+
+
+def test_warmnet_reference_gradient_is_the_derivative():
+    """oracle/warmnet_ref.py (the checker of the GPU network kernels) is pinned
+    by central differences of its own loss (every gradient entry agrees) and by
+    the loss being the mean squared distance to the sign-fixed eigenvectors."""
+    import warmnet_ref
+    rng = np.random.default_rng(0)
+    n, h, B = 3, 5, 4
+    din, dout, offs = warmnet_ref.dims(n, h)
+    p = rng.normal(size=offs[-1]) * 0.5
+    G = rng.normal(size=(B, n, n))
+    A = G + np.swapaxes(G, 1, 2)
+    V = np.linalg.eigh(A)[1]
+    loss, g = warmnet_ref.loss_and_grad(p, A, V, n, h)
+    eps = 1e-6
+    fd = np.empty_like(p)
+    for i in range(len(p)):
+        e = np.zeros_like(p)
+        e[i] = eps
+        fd[i] = (warmnet_ref.loss_and_grad(p + e, A, V, n, h)[0] - warmnet_ref.loss_and_grad(p - e, A, V, n, h)[0]) / (2 * eps)
+    assert np.max(np.abs(fd - g)) <= 1e-6 * max(1.0, np.abs(g).max())
+    # the loss is the mean squared distance to the sign-fixed eigenvectors
+    _, _, _, y = warmnet_ref.forward(p, A, n, h)
+    T = warmnet_ref.sign_fixed(V)
+    assert abs(loss - np.mean((y - T) ** 2)) <= 1e-14
+    assert np.all(np.abs(T.reshape(B, n, n)).argmax(axis=2) >= 0)
+    Tm = np.swapaxes(T.reshape(B, n, n), 1, 2)
+    big = np.take_along_axis(Tm, np.abs(Tm).argmax(axis=1)[:, None, :], axis=1)
+    assert np.all(big > 0)
