@@ -575,6 +575,11 @@ def bench_mixed(q0, plain_ms, steps, warmup):
     bad = int((info != 0).sum().item())
     del a, w
     torch.cuda.empty_cache()
+    ms16, _, a, w, info = time_solver(q0, eg.MIXED | eg.FLAG_MIXED_WY16, steps, warmup)
+    acc16 = accuracy_all(q0, a, w)
+    bad16 = int((info != 0).sum().item())
+    del a, w
+    torch.cuda.empty_cache()
     q32 = q0.float().contiguous()
     ms32, _, a32, w32, _ = time_solver(q32, 0, steps, warmup)
     del q32, a32, w32
@@ -582,6 +587,9 @@ def bench_mixed(q0, plain_ms, steps, warmup):
     return {"value": L / (ms * 1e-3), "unit": "matrices/s", "ms_per_step": ms, "vs_plain": plain_ms / ms,
             "fp32_solve_ms": ms32, "refinement_ms": ms - ms32, "nonconverged": bad, "accuracy_all": acc,
             "relative_residual_max": acc["max_err1_rel"],
+            "fp16_blocked_back_transform": {"ms_per_step": ms16, "value": L / (ms16 * 1e-3),
+                                            "vs_plain": plain_ms / ms16, "nonconverged": bad16,
+                                            "accuracy_all": acc16},
             "gemm_roofline": gemm_f64_roofline(2048, n)}
 
 
@@ -826,6 +834,10 @@ def main():
                     help="run only the n=256 FP64 line (for profiling its kernels)")
     ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start line")
     ap.add_argument("--warmstart-only", action="store_true", help="print only the learned warm-start line")
+    ap.add_argument("--ws-train-steps", type=int, default=50, help="warm-start line: training steps")
+    ap.add_argument("--ws-total-steps", type=int, default=200, help="warm-start line: stream length")
+    ap.add_argument("--ws-eval-steps", type=int, nargs="+", default=[50, 100, 150, 199],
+                    help="warm-start line: evaluation steps")
     ap.add_argument("--mixed-only", action="store_true",
                     help="print only the n=256 FP64 mixed-precision line (and the plain time)")
     ap.add_argument("--uneven-count", type=int, default=100000)
@@ -842,7 +854,9 @@ def main():
     import eigeng_batched as eg
 
     if args.warmstart_only:
-        print(json.dumps({"warmstart": bench_warmstart()}))
+        print(json.dumps({"warmstart": bench_warmstart(train_steps=args.ws_train_steps,
+                                                       total_steps=args.ws_total_steps,
+                                                       eval_steps=tuple(args.ws_eval_steps))}))
         return
     if args.mixed_only:
         import torch

@@ -16,6 +16,13 @@
 //            float accumulate
 //   GK_BF16  float in rounded (nearest even) to bfloat16, v_mfma_f32_16x16x16bf16_1k,
 //            float accumulate
+//   GK_F16X3 float in split into two halves, x = hi + lo / 2^11 with hi = half(x) and
+//            lo = half((x - hi) 2^11), and multiplied as hi*hi + (hi*lo + lo*hi) / 2^11 on
+//            v_mfma_f32_16x16x16f16 (three products, two FP32 accumulators): FP16
+//            matrix cores at close to FP32 accuracy (the error-compensated splitting
+//            of Ootomo & Yokota, "Recovering single precision accuracy from Tensor
+//            Cores while surpassing the FP32 theoretical peak performance", IJHPCA
+//            2022); |x| must stay below 65504
 //
 // Shape: a 256-thread workgroup computes a 64 x 64 tile of one C_b; wave q owns
 // the 32 x 32 quarter (q & 1, q >> 1), i.e. 2 x 2 MFMA tiles of 16 x 16.  The
@@ -41,7 +48,8 @@
 
 namespace eigmi {
 
-enum GemmKind { GK_F64 = 0, GK_F32 = 1, GK_F16 = 2, GK_BF16 = 3 };
+enum GemmKind { GK_F64 = 0, GK_F32 = 1, GK_F16 = 2, GK_BF16 = 3, GK_F16X3 = 4 };
+constexpr float GM_X3_SCALE = 2048.0f;   // 2^11: the low half's scale
 
 struct GemmArgs {
   int M, N, K, batch;
@@ -89,6 +97,23 @@ template <> struct GemmT<GK_BF16> {
     if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even
     return (unsigned short)(u >> 16);
+  }
+};
+
+struct HalfPair {
+  _Float16 hi, lo;
+};
+template <> struct GemmT<GK_F16X3> {
+  typedef float S;
+  typedef HalfPair L;
+  typedef float Acc;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ L cvt(S x)
+  {
+    L r;
+    r.hi = (_Float16)x;
+    r.lo = (_Float16)((x - (float)r.hi) * GM_X3_SCALE);
+    return r;
   }
 };
 
@@ -150,6 +175,37 @@ template <> struct GemmMma<GK_F16> {
     acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x16f16(a0, b1, acc[0][1], 0, 0, 0);
     acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x16f16(a1, b0, acc[1][0], 0, 0, 0);
     acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x16f16(a1, b1, acc[1][1], 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+template <> struct GemmMma<GK_F16X3> {
+  typedef GemmT<GK_F16X3>::v4 v4;
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  // acc gets hi*hi, acc2 the cross terms (scaled by 2^11)
+  static __device__ __forceinline__ void chunk(const HalfPair (*As)[GM_TILE + GM_PAD],
+                                               const HalfPair (*Bs)[GM_TILE + GM_PAD], int r0, int c0, int m,
+                                               int g, v4 (&acc)[2][2], v4 (&acc2)[2][2])
+  {
+    h4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const HalfPair a = As[4 * g + i][r0 + 16 * x + m], b = Bs[4 * g + i][c0 + 16 * x + m];
+        ah[x][i] = a.hi;
+        al[x][i] = a.lo;
+        bh[x][i] = b.hi;
+        bl[x][i] = b.lo;
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+        acc2[x][y] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[x], bl[y], acc2[x][y], 0, 0, 0);
+        acc2[x][y] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[x], bh[y], acc2[x][y], 0, 0, 0);
+      }
   }
   static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
@@ -260,13 +316,13 @@ __global__ __launch_bounds__(GM_THREADS) void eig_gemm_kernel(GemmArgs g)
     }
   };
 
-  v4 acc[2][2];
+  v4 acc[2][2], acc2[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[x][y][r] = 0;
+      for (int r = 0; r < 4; ++r) acc[x][y][r] = acc2[x][y][r] = 0;
 
   S ra[4], rb[4];
   loadA(0, ra);
@@ -282,7 +338,8 @@ __global__ __launch_bounds__(GM_THREADS) void eig_gemm_kernel(GemmArgs g)
       loadA(k0 + GM_KC, ra);
       loadB(k0 + GM_KC, rb);
     }
-    GemmMma<KIND>::chunk(As, Bs, wr, wc, m, gg, acc);
+    if constexpr (KIND == GK_F16X3) GemmMma<KIND>::chunk(As, Bs, wr, wc, m, gg, acc, acc2);
+    else GemmMma<KIND>::chunk(As, Bs, wr, wc, m, gg, acc);
     __syncthreads();
   }
 
@@ -301,6 +358,7 @@ __global__ __launch_bounds__(GM_THREADS) void eig_gemm_kernel(GemmArgs g)
         const int j = col0 + wc + 16 * y + m;
         if (i < g.M && j < g.N) {
           Acc v = alpha * acc[x][y][r];
+          if constexpr (KIND == GK_F16X3) v = alpha * (acc[x][y][r] + acc2[x][y][r] * (1.0f / GM_X3_SCALE));
           if (D) v += beta * (Acc)D[i + (size_t)j * g.ldd];
           C[i + (size_t)j * g.ldc] = (S)v;
         }

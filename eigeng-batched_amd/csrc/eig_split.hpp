@@ -221,7 +221,7 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
       }
     }
   } else {
-    // implicit shift (imtql2_tiled.hpp:235-321) as one flat loop: every pass
+    // implicit shift (imtql2_tiled.hpp:43-134) as one flat loop: every pass
     // runs one sweep of the lane's current eigenvalue l, moving l on past the
     // converged ones first.  Nested loops over l and its sweeps would make the
     // lanes of a wave wait for each other at every l (a wave's lanes leave a

@@ -587,7 +587,10 @@ template <typename T, int NCH> struct ApWaves { static constexpr int value = 4; 
 
 // LIST: the uneven-batch form (u), one workgroup per (slot, row block of
 // order nmx); otherwise (matrix, row block) of a uniform batch of order n.
-template <typename T, int NCH, int CW, bool LIST = false>
+// SKIPQ: start from the identity instead of Q (phase A skipped): the result is
+// the tridiagonal's eigenvector matrix Z, for a separate back-transformation
+// (eig_wy.hpp).
+template <typename T, int NCH, int CW, bool LIST = false, bool SKIPQ = false>
 __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWaves<T, NCH>::value))) void eig_apply_large_kernel(
     int L, int nm, int n, T *__restrict__ a, const T *__restrict__ W, const T *__restrict__ bws,
     const typename vec2<T>::type *__restrict__ rotlog, int rot_cap, const int *__restrict__ swlog, int sw_cap,
@@ -632,7 +635,7 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
   T unext = (lane < CW && c0 + lane < n - 1) ? Wm[c0 + lane + (size_t)(n - 1) * n] : T(0);
   T unext2 = (lane < CW && n >= 3 && c0 + lane < n - 2) ? Wm[c0 + lane + (size_t)(n - 2) * n] : T(0);
   __syncthreads();
-  for (int c = n - 1; c >= 1; --c) {
+  for (int c = SKIPQ ? 0 : n - 1; c >= 1; --c) {
     const T ucur = (c0 + lane < c) ? unext : T(0);
     unext = unext2;
     if (c > 2 && lane < CW) unext2 = (c0 + lane < c - 2) ? Wm[c0 + lane + (size_t)(c - 2) * n] : T(0);

@@ -382,11 +382,11 @@ __device__ __forceinline__ void wave_solve(T *__restrict__ A, int nm, int n, T *
       }
     }
   } else {
-    // implicit shift (imtql2_tiled.hpp:235-321)
+    // implicit shift (imtql2_tiled.hpp:43-134)
     for (int l = 0; l < n && !err; ++l) {
       for (int itr = 0;; ++itr) {
         // itr sweeps done; the reference stops after max_sweep+1 sweeps without
-        // a final test (imtql2_tiled.hpp:242,319)
+        // a final test (imtql2_tiled.hpp:49,126)
         if (itr > QL_MAX_SWEEP) { err = l + 1; break; }
         bool cand[NW];
 #pragma unroll

@@ -24,6 +24,7 @@
 #include "eig_layout.hpp"
 #include "eig_warm.hpp"
 #include "eig_warmnet.hpp"
+#include "eig_wy.hpp"
 
 #include <stdio.h>
 #include <vector>
@@ -64,6 +65,10 @@ constexpr int FLAG_UNSORTED = 0x40000000;
 // FP64 refinement on MFMA (eig_refine.hpp) with the plain FP64 solver as the
 // fallback for matrices the refinement does not accept
 constexpr int SOLVER_MIXED = 3;
+// with SOLVER_MIXED, 64 < n <= 256: the FP32 stage takes the large-order split
+// path with the back-transformation as blocked reflectors on the matrix cores
+// (FP16-split products, eig_wy.hpp) instead of forming Q on the vector units
+constexpr int FLAG_MIXED_WY16 = 0x8000000;
 // bits 28-29 of `solver`: refinement iterations (0: the default, 3)
 constexpr int MIX_IT_SHIFT = 28;
 constexpr int MIX_DEFAULT_ITERS = 3;
@@ -193,26 +198,31 @@ struct RefineLayout {
 // Mixed-precision path (FP64 interface): the FP32 copy of the batch, its
 // eigenvalues and status, the FP32 solver's own workspace, then the refinement
 // region.
+// wy: the FP32 stage is the large-order split path with the blocked
+// back-transformation, whose buffers follow the refinement region.
+size_t wy_bytes(int lc, int n, int tb);
 struct MixedLayout {
-  size_t a32, w32, info32, ws32, ref, total;
-  MixedLayout(int L, int n)
+  size_t a32, w32, info32, ws32, ref, wyr, total;
+  MixedLayout(int L, int n, bool wy)
   {
     const size_t nn = (size_t)n * n;
     a32 = 0;
     w32 = a32 + al256((size_t)L * nn * 4);
     info32 = w32 + al256((size_t)L * n * 4);
     ws32 = info32 + al256((size_t)L * 4);
-    ref = ws32 + al256(workspace_bytes(L, n, n, n, 4, 0));
-    total = ref + RefineLayout(L, n).total;
+    ref = ws32 + al256(workspace_bytes(L, n, n, n, 4, wy ? FLAG_SPLIT : 0));
+    wyr = ref + al256(RefineLayout(L, n).total);
+    total = wyr + (wy ? wy_bytes(L < LCHUNK_LARGE ? L : LCHUNK_LARGE, n, 4) : 0);
   }
 };
+bool mixed_wy(int n, int solver) { return (solver & FLAG_MIXED_WY16) && n > 64; }
 
 size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
 {
   (void)nm; (void)m;
   if ((solver & 0xff) == SOLVER_MIXED) {
     if (tbytes != 8 || n > 256) return 0;
-    return WS_HEAD + MixedLayout(L, n).total;
+    return WS_HEAD + MixedLayout(L, n, mixed_wy(n, solver)).total;
   }
   if (use_split(L, n, solver) || use_split128(L, n, tbytes, solver))
     return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
@@ -328,18 +338,89 @@ hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver
   return hipGetLastError();
 }
 
-template <typename T, int NCH, int CW>
+template <typename T, int NCH, int CW, bool SKIPQ = false>
 void launch_apply_large(unsigned grid, int L, int nm, int n, T *a, const T *Wm, const T *bw,
                         const typename vec2<T>::type *rot, int rcap, const int *sw, int scap, const int *nsw,
                         const int *st, const int *slot, hipStream_t s)
 {
-  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH, CW>), dim3(grid), dim3(64 * NCH), 0, s, L, nm, n, a, Wm, bw,
-                     rot, rcap, sw, scap, nsw, st, slot);
+  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH, CW, false, SKIPQ>), dim3(grid), dim3(64 * NCH), 0, s, L, nm, n,
+                     a, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, UvSplitArgs{});
 }
 
 template <typename T>
+size_t wy_bytes(int lc, int n, int tb);
+hipError_t gemm_launch(int kind, const GemmArgs &g, hipStream_t s);
+
+// Blocked back-transformation buffers for a chunk of lc matrices: scaled betas,
+// U^T U and T per block, and the two panel products.
+struct WyLayout {
+  size_t bwy, g, t, p1, p2, total;
+  WyLayout(int lc, int n, int tb)
+  {
+    bwy = 0;
+    g = bwy + al256((size_t)lc * n * tb);
+    t = g + al256((size_t)lc * WY_NB * WY_NB * tb);
+    p1 = t + al256((size_t)lc * WY_NB * WY_NB * tb);
+    p2 = p1 + al256((size_t)lc * WY_NB * n * tb);
+    total = p2 + al256((size_t)lc * WY_NB * n * tb);
+  }
+};
+size_t wy_bytes(int lc, int n, int tb) { return WyLayout(lc, n, tb).total; }
+
+// V (ld nm) <- H_{n-1} ... H_1 V with the reflectors of the large-order
+// tridiagonalisation (Wm, bw); matrices with status set keep V (eig_wy.hpp).
+hipError_t wy_backtransform(int cur, int nm, int n, float *Wm, const float *bw, const int *st, float *V, char *wyb,
+                            hipStream_t s)
+{
+  const WyLayout ly(cur, n, 4);
+  float *bwy = (float *)(wyb + ly.bwy), *G = (float *)(wyb + ly.g), *Tm = (float *)(wyb + ly.t);
+  float *P1 = (float *)(wyb + ly.p1), *P2 = (float *)(wyb + ly.p2);
+  const long long cols = (long long)cur * n;
+  hipLaunchKernelGGL(wy_prepare_kernel<float>, dim3((unsigned)((cols + 3) / 4 < 65536 ? (cols + 3) / 4 : 65536)),
+                     dim3(256), 0, s, cur, n, Wm, bw, bwy, st);
+  const long long nn = (long long)n * n;
+  hipError_t err;
+  for (int c0 = 0; c0 < n; c0 += WY_NB) {
+    const int nb = (n - c0) < WY_NB ? (n - c0) : WY_NB;
+    GemmArgs g{};
+    g.batch = cur;
+    g.alpha = 1.0;
+    // G = U^T U
+    g.M = g.N = nb; g.K = n; g.transa = 1; g.transb = 0;
+    g.A = Wm + (size_t)c0 * n; g.lda = n; g.sa = nn;
+    g.B = Wm + (size_t)c0 * n; g.ldb = n; g.sb = nn;
+    g.C = G; g.ldc = WY_NB; g.sc = WY_NB * WY_NB;
+    if ((err = gemm_launch(GK_F32, g, s)) != hipSuccess) return err;
+    hipLaunchKernelGGL(wy_t_kernel<float>, dim3((unsigned)(cur < 65536 ? cur : 65536)), dim3(64), 0, s, cur, n, c0,
+                       nb, (const float *)G, (const float *)bwy, Tm);
+    // P1 = U^T V
+    g.M = nb; g.N = n; g.K = n; g.transa = 1;
+    g.B = V; g.ldb = nm; g.sb = (long long)nm * n;
+    g.C = P1; g.ldc = WY_NB; g.sc = (long long)WY_NB * n;
+    if ((err = gemm_launch(GK_F16X3, g, s)) != hipSuccess) return err;
+    // P2 = T P1
+    g.M = nb; g.N = n; g.K = nb; g.transa = 0;
+    g.A = Tm; g.lda = WY_NB; g.sa = WY_NB * WY_NB;
+    g.B = P1; g.ldb = WY_NB; g.sb = (long long)WY_NB * n;
+    g.C = P2; g.ldc = WY_NB; g.sc = (long long)WY_NB * n;
+    if ((err = gemm_launch(GK_F32, g, s)) != hipSuccess) return err;
+    // V += U P2
+    g.M = n; g.N = n; g.K = nb;
+    g.A = Wm + (size_t)c0 * n; g.lda = n; g.sa = nn;
+    g.B = P2; g.ldb = WY_NB; g.sb = (long long)WY_NB * n;
+    g.D = V; g.ldd = nm; g.sd = (long long)nm * n; g.beta = 1.0;
+    g.C = V; g.ldc = nm; g.sc = (long long)nm * n;
+    if ((err = gemm_launch(GK_F16X3, g, s)) != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+// wyb (FP32 only, may be null): run the back-transformation as blocked
+// reflectors on the matrix cores (the apply kernel then only replays the QL
+// rotations from the identity) in a region of WyLayout(chunk, n, 4) bytes.
+template <typename T>
 hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int solver, int kop, void *wk,
-                              hipStream_t s)
+                              hipStream_t s, char *wyb = nullptr)
 {
   typedef typename vec2<T>::type T2;
   const int lc = L < LCHUNK_LARGE ? L : LCHUNK_LARGE;
@@ -409,6 +490,19 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
     if (!(ph & FLAG_PH_APPLY)) continue;
     const unsigned ag = (unsigned)cur * (unsigned)((n + 63) / 64);
     constexpr int CWS = 32;  // chunk width
+    if constexpr (sizeof(T) == 4) {
+      if (wyb) {
+        if (n <= 128)
+          launch_apply_large<T, 128 / CWS, CWS, true>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+        else
+          launch_apply_large<T, 256 / CWS, CWS, true>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+        hipError_t e2 = wy_backtransform(cur, nm, n, Wm, bw, st, ac, wyb, s);
+        if (e2 != hipSuccess) return e2;
+        hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(tg), dim3(GL_THREADS), 0, s, cur, nm, n, ac, wc,
+                           ic, sv, Wm, (const int *)st);
+        continue;
+      }
+    }
     if (n <= 128)
       launch_apply_large<T, 128 / CWS, CWS>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else if (n <= 256)
@@ -437,7 +531,7 @@ size_t gemm_extent(long long rows, long long cols, long long ld, long long strid
 // Validates and launches eig_gemm_kernel<kind>: one workgroup per 64 x 64 tile.
 hipError_t gemm_launch(int kind, const GemmArgs &g, hipStream_t s)
 {
-  if (kind < GK_F64 || kind > GK_BF16) return hipErrorInvalidValue;
+  if (kind < GK_F64 || kind > GK_F16X3) return hipErrorInvalidValue;
   if (g.M < 1 || g.N < 1 || g.K < 1 || g.batch < 0 || !g.A || !g.B || !g.C) return hipErrorInvalidValue;
   if (g.batch == 0) return hipSuccess;
   const long long ar = g.transa ? g.K : g.M, ac = g.transa ? g.M : g.K;
@@ -464,7 +558,8 @@ hipError_t gemm_launch(int kind, const GemmArgs &g, hipStream_t s)
   case GK_F64: hipLaunchKernelGGL((eig_gemm_kernel<GK_F64>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
   case GK_F32: hipLaunchKernelGGL((eig_gemm_kernel<GK_F32>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
   case GK_F16: hipLaunchKernelGGL((eig_gemm_kernel<GK_F16>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
-  default: hipLaunchKernelGGL((eig_gemm_kernel<GK_BF16>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
+  case GK_BF16: hipLaunchKernelGGL((eig_gemm_kernel<GK_BF16>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
+  default: hipLaunchKernelGGL((eig_gemm_kernel<GK_F16X3>), dim3((unsigned)total), dim3(GM_THREADS), 0, s, g); break;
   }
   return hipGetLastError();
 }
@@ -531,7 +626,8 @@ hipError_t run_mixed(int L, int nm, int n, int m, double *a, double *w, void *wk
   if (n > 256) return hipErrorNotSupported;
   const int iters = ((solver >> MIX_IT_SHIFT) & 3) ? ((solver >> MIX_IT_SHIFT) & 3) : MIX_DEFAULT_ITERS;
   const int sorted = (solver & FLAG_UNSORTED) ? 0 : 1;
-  const MixedLayout lay(L, n);
+  const bool wy = mixed_wy(n, solver);
+  const MixedLayout lay(L, n, wy);
   const RefineLayout ly(L, n);
   char *base = (char *)wk + WS_HEAD;
   char *rb = base + lay.ref;
@@ -548,7 +644,11 @@ hipError_t run_mixed(int L, int nm, int n, int m, double *a, double *w, void *wk
                      s, L, n, (const double *)a, (long long)nm, (long long)nm * n, a32, (long long)n, (long long)nn);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  err = run<float>(L, n, n, n, a32, w32, base + lay.ws32, s, SOLVER_IMTQL2, info32);
+  if (wy)
+    err = launch_split_large<float>(L, n, n, a32, w32, info32, SOLVER_IMTQL2 | FLAG_SPLIT,
+                                    kopt_make(SOLVER_IMTQL2, false, 0), base + lay.ws32, s, base + lay.wyr);
+  else
+    err = run<float>(L, n, n, n, a32, w32, base + lay.ws32, s, SOLVER_IMTQL2, info32);
   if (err != hipSuccess) return err;
   for (long long c0 = 0; c0 < L; c0 += ly.lc) {
     const int cur = (int)((L - c0) < ly.lc ? (L - c0) : ly.lc);

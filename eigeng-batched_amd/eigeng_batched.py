@@ -24,8 +24,11 @@ IMTQL2, TQL2, JACOBI = 0, 1, 2
 # plain FP64 solver as fallback; bits 28-29 set the refinement iterations (0: 3)
 MIXED = 3
 MIX_ITERS_SHIFT = 28
+# with MIXED, 64 < n <= 256: the FP32 stage's back-transformation as blocked
+# reflectors on the FP16 matrix cores (error-compensated split products)
+FLAG_MIXED_WY16 = 0x8000000
 # batched products (eigen_GPU_batch_gemm): matrix-core operand kinds
-GEMM_F64, GEMM_F32, GEMM_F16, GEMM_BF16 = 0, 1, 2, 3
+GEMM_F64, GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3 = 0, 1, 2, 3, 4
 # tuning flags OR-ed into the solver id of the _x entry points
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
 FLAG_QL_LDS = 0x800

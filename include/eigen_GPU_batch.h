@@ -70,6 +70,14 @@ enum eigen_GPU_batch_solver {
 /* bits 28-29 of `solver` with EIGEN_GPU_BATCH_MIXED: refinement iterations
  * (1..3; 0 selects the default, 3). */
 #define EIGEN_GPU_BATCH_MIXED_ITERS_SHIFT 28
+/* with EIGEN_GPU_BATCH_MIXED and 64 < n <= 256: the FP32 stage takes the
+ * large-order split path and back-transforms with blocked reflectors (compact
+ * WY, blocks of 64: V <- V + U (T (U^T V))) on the FP16 matrix cores with FP32
+ * accumulation (error-compensated split operands), the QL rotations replayed
+ * from the identity; the FP64 refinement then wins back FP64 accuracy (the
+ * reference's blocked back-transformation reduces with shuffles,
+ * hhtr2sy.hpp:144-202). */
+#define EIGEN_GPU_BATCH_MIXED_WY16 0x8000000
 
 /*
  * Tuning bits OR-ed into `solver` of the _x entry points and of
@@ -376,8 +384,10 @@ int eigen_GPU_batch_interleaved_DP(int L, int nm, int n, int m, double *a, doubl
  * K x N, leading dimensions and batch strides in elements.  kind: 0 FP64
  * operands (v_mfma_f64_16x16x4f64), 1 FP32 (v_mfma_f32_16x16x4f32), 2 FP32
  * storage multiplied as FP16 (v_mfma_f32_16x16x16f16), 3 FP32 storage
- * multiplied as bfloat16 (v_mfma_f32_16x16x16bf16_1k); FP32 accumulation for
- * kinds 1-3.  D may be NULL (beta ignored) or equal to C (same ldc); C must not
+ * multiplied as bfloat16 (v_mfma_f32_16x16x16bf16_1k), 4 FP32 storage split
+ * into two halves (hi + lo / 2^11) and multiplied as three FP16 products with
+ * FP32 accumulation, close to FP32 accuracy (error-compensated splitting,
+ * Ootomo & Yokota 2022; |values| < 65504); FP32 accumulation for kinds 1-4.  D may be NULL (beta ignored) or equal to C (same ldc); C must not
  * overlap A or B.  Returns hipSuccess (batch == 0: nothing launched),
  * hipErrorInvalidValue, or the launch error.
  */

@@ -219,7 +219,7 @@ def test_gemm_validates_arguments_without_a_gpu():
         return f(a["kind"], a["ta"], a["tb"], a["M"], a["N"], a["K"], 1.0, a["A"], a["lda"], a["sa"],
                  a["B"], a["ldb"], a["sb"], 0.0, None, 0, 0, a["C"], a["ldc"], a["sc"], a["batch"], None)
     assert call() == 0
-    for bad in (dict(kind=4), dict(kind=-1), dict(M=0), dict(K=0), dict(batch=-1), dict(lda=7, batch=1),
+    for bad in (dict(kind=5), dict(kind=-1), dict(M=0), dict(K=0), dict(batch=-1), dict(lda=7, batch=1),
                 dict(ta=1, lda=7, batch=1), dict(ldb=7, batch=1), dict(ldc=7, batch=1),
                 dict(C=A + 8, batch=1), dict(B=C, batch=1), dict(sa=-1, batch=1)):
         assert call(**bad) == 1, bad

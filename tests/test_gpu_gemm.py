@@ -29,6 +29,13 @@ def ref_and_bound(A, B, ta, tb, kind):
         a, b = A.half().double(), B.half().double()
     elif kind == eg.GEMM_BF16:
         a, b = bf16_round(A).double(), bf16_round(B).double()
+    elif kind == eg.GEMM_F16X3:
+        # the split operands: hi + lo / 2^11, both halves rounded
+        def split(X):
+            hi = X.half().float()
+            lo = ((X - hi) * 2048.0).half().float()
+            return hi.double() + lo.double() / 2048.0
+        a, b = split(A), split(B)
     # row-major views of the column-major matrices: Am[b] = a[b].T
     Am = a.transpose(1, 2)
     Bm = b.transpose(1, 2)
@@ -39,10 +46,13 @@ def ref_and_bound(A, B, ta, tb, kind):
     K = opA.shape[2]
     eps = 2.0 ** -52 if kind == eg.GEMM_F64 else 2.0 ** -23
     bound = 4.0 * K * eps * Cabs + 1e-300
+    if kind == eg.GEMM_F16X3:
+        # the dropped lo*lo term: 2^-22 of |A||B| per product, plus FP32 accumulation
+        bound = bound + 2.0 ** -21 * Cabs
     return C.transpose(1, 2), bound.transpose(1, 2)
 
 
-KINDS = [eg.GEMM_F64, eg.GEMM_F32, eg.GEMM_F16, eg.GEMM_BF16]
+KINDS = [eg.GEMM_F64, eg.GEMM_F32, eg.GEMM_F16, eg.GEMM_BF16, eg.GEMM_F16X3]
 SHAPES = [(1, 1, 1), (17, 33, 5), (64, 64, 64), (100, 70, 130), (256, 256, 256), (5, 200, 48)]
 
 

@@ -89,6 +89,18 @@ def test_mixed_clustered_qdq(n):
     print(f"[mixed] n={n} max relative residual {rel.max():.3e}", flush=True)
 
 
+@pytest.mark.parametrize("n", [65, 100, 200, 256])
+def test_mixed_with_the_fp16_blocked_back_transform(n):
+    """The FP32 stage with the back-transformation as blocked reflectors on the
+    FP16 matrix cores (EIGEN_GPU_BATCH_MIXED_WY16): the same bar, clustered
+    spectrum, enough matrices for the large-order split path."""
+    print(f"[mixed] fp16 blocked back-transform n={n}", flush=True)
+    L = 520
+    lam = wl.clustered_uniform(L, n, frac=0.05, gap=1e-8, seed=90 + n)
+    a0 = wl.qdq(lam, dtype=np.float64, seed=91 + n)
+    check_mixed(a0, iters_bits=eg.FLAG_MIXED_WY16, oracle_sample=2)
+
+
 def test_mixed_large_clusters_take_the_fallback():
     """A triple eigenvalue (resolved by Rayleigh-Ritz), a 12-fold one and the
     identity (clusters beyond the Rayleigh-Ritz size: the plain FP64 fallback)."""
