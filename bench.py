@@ -514,6 +514,7 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
     tuned2 = tuned_line(q0, ms2, steps, warmup) if tuned else None
     vend = vendor_baseline(q0[:512].contiguous(), reps=1) if vendor else None
     mixed = bench_mixed(q0, ms2, steps, warmup)
+    chunks2 = chunk_tradeoff(q0, (0, 1), steps=steps, warmup=warmup)
     ph2, a2, w2, _ = time_phases(q0, 2, 1)
     del q0, a2, w2
     torch.cuda.empty_cache()
@@ -523,6 +524,7 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
     return {"workload": "L=8192 n=256 FP64, A = Q diag(lam) Q^T, lam ~ U[-1,1], 5% pairs split by 1e-8",
             "value": L2 / (ms2 * 1e-3), "unit": "matrices/s", "ms_per_step": ms2, "nonconverged": bad,
             "accuracy_all": acc2, "tuned": tuned2, "vendor_baseline_512": vend, "mixed": mixed,
+            "chunk_tradeoff": chunks2,
             "roofline": {"bound": "valu", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
@@ -693,6 +695,24 @@ def bench_warmstart(L=65536, n=48, p=300, rho=1.05, hidden=512, train_steps=50, 
             "vs_plain": {k: mean(k, "vs_plain") for k in ("network", "identity", "previous_step")},
             "mean_sweeps": {k: mean(k, "mean_sweeps") for k in ("network", "identity", "previous_step")},
             "evals": evals}
+
+
+def chunk_tradeoff(a0, shifts, steps=5, warmup=1):
+    """Time and workspace of the same solve with the split path's batch chunk
+    divided by 2^shift (tuning bits 18-19): the memory / time trade-off."""
+    import torch
+    import eigeng_batched as eg
+    L, n, nm = a0.shape
+    out = {}
+    for sh in shifts:
+        bits = sh << eg.CHUNK_SHIFT
+        ms, _, a, w, info = time_solver(a0, bits, steps, warmup)
+        out[f"chunk/{1 << sh}"] = {"ms_per_step": ms, "value": L / (ms * 1e-3),
+                                   "workspace_bytes": eg.bufsize(L, nm, n, n, a0.element_size(), bits),
+                                   "nonconverged": int((info != 0).sum().item())}
+        del a, w
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_n128(steps, warmup):
@@ -944,6 +964,8 @@ def main():
         "accuracy": acc,
     }
     out["accuracy_all"] = accuracy_all(a0, a, w)
+    if n <= 64 and L >= 2048:
+        out["chunk_tradeoff"] = chunk_tradeoff(a0, (0, 1, 2, 3), steps=args.steps, warmup=args.warmup)
     if not args.no_tuned:
         out["tuned"] = tuned_line(a0, ms, args.steps, args.warmup)
     if not args.no_vendor:

@@ -102,6 +102,11 @@ struct SplitLayout {
 // Large-order split path (orders beyond the register kernels): workspace
 // regions for a chunk of lc matrices.
 constexpr int LCHUNK_LARGE = 8192;
+// bits 18-19 of `solver`: the split paths' chunk divided by 2, 4 or 8 (a smaller
+// workspace for a longer call: the latency-bound QL recurrence runs once per chunk)
+constexpr int CHUNK_SHIFT = 18;
+int lchunk(int solver) { return LCHUNK >> ((solver >> CHUNK_SHIFT) & 3); }
+int lchunk_large(int solver) { return LCHUNK_LARGE >> ((solver >> CHUNK_SHIFT) & 3); }
 constexpr int SPLIT_LARGE_MIN_L = 512;
 struct SplitLargeLayout {
   size_t wm, de, bw, rot, sw, nsw, status, slot, total;
@@ -177,9 +182,10 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver);
 struct RefineLayout {
   size_t af, pe, x0, x1, sm, gm, inf, st, why, emx, total;
   int lc;
-  RefineLayout(int L, int n)
+  RefineLayout(int L, int n, int solver = 0)
   {
-    lc = L < LCHUNK_REF ? L : LCHUNK_REF;
+    const int cap = LCHUNK_REF >> ((solver >> CHUNK_SHIFT) & 3);
+    lc = L < cap ? L : cap;
     const size_t nn = (size_t)n * n;
     af = 0;
     pe = af + al256((size_t)lc * nn * 8);
@@ -203,16 +209,17 @@ struct RefineLayout {
 size_t wy_bytes(int lc, int n, int tb);
 struct MixedLayout {
   size_t a32, w32, info32, ws32, ref, wyr, total;
-  MixedLayout(int L, int n, bool wy)
+  // cb: the caller's chunk bits, passed on to the FP32 stage and the refinement
+  MixedLayout(int L, int n, bool wy, int cb = 0)
   {
     const size_t nn = (size_t)n * n;
     a32 = 0;
     w32 = a32 + al256((size_t)L * nn * 4);
     info32 = w32 + al256((size_t)L * n * 4);
     ws32 = info32 + al256((size_t)L * 4);
-    ref = ws32 + al256(workspace_bytes(L, n, n, n, 4, wy ? FLAG_SPLIT : 0));
-    wyr = ref + al256(RefineLayout(L, n).total);
-    total = wyr + (wy ? wy_bytes(L < LCHUNK_LARGE ? L : LCHUNK_LARGE, n, 4) : 0);
+    ref = ws32 + al256(workspace_bytes(L, n, n, n, 4, (wy ? FLAG_SPLIT : 0) | cb));
+    wyr = ref + al256(RefineLayout(L, n, cb).total);
+    total = wyr + (wy ? wy_bytes(L < lchunk_large(cb) ? L : lchunk_large(cb), n, 4) : 0);
   }
 };
 bool mixed_wy(int n, int solver) { return (solver & FLAG_MIXED_WY16) && n > 64; }
@@ -222,13 +229,13 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
   (void)nm; (void)m;
   if ((solver & 0xff) == SOLVER_MIXED) {
     if (tbytes != 8 || n > 256) return 0;
-    return WS_HEAD + MixedLayout(L, n, mixed_wy(n, solver)).total;
+    return WS_HEAD + MixedLayout(L, n, mixed_wy(n, solver), solver & (3 << CHUNK_SHIFT)).total;
   }
   if (use_split(L, n, solver) || use_split128(L, n, tbytes, solver))
-    return WS_HEAD + SplitLayout(L < LCHUNK ? L : LCHUNK, n, tbytes).total;
+    return WS_HEAD + SplitLayout(L < lchunk(solver) ? L : lchunk(solver), n, tbytes).total;
   if (n > GL_NMAX) return 0;
   if (use_split_large(L, n, tbytes, solver))
-    return WS_HEAD + SplitLargeLayout(L < LCHUNK_LARGE ? L : LCHUNK_LARGE, n, tbytes).total;
+    return WS_HEAD + SplitLargeLayout(L < lchunk_large(solver) ? L : lchunk_large(solver), n, tbytes).total;
   if (use_wave_kernel(n, tbytes)) return WS_HEAD;
   return WS_HEAD + (size_t)global_grid(L) * (size_t)n * (size_t)n * (size_t)tbytes;
 }
@@ -250,7 +257,7 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
 {
   typedef typename vec2<T>::type T2;
   constexpr int G = GroupShape<NB>::G, NW = GroupShape<NB>::NW;
-  const int lc = L < LCHUNK ? L : LCHUNK;
+  const int lc = L < lchunk(solver) ? L : lchunk(solver);
   const SplitLayout lay(lc, n, (int)sizeof(T));
   char *base = (char *)wk + WS_HEAD;
   T *q = (T *)(base + lay.q), *de = (T *)(base + lay.de);
@@ -423,7 +430,7 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
                               hipStream_t s, char *wyb = nullptr)
 {
   typedef typename vec2<T>::type T2;
-  const int lc = L < LCHUNK_LARGE ? L : LCHUNK_LARGE;
+  const int lc = L < lchunk_large(solver) ? L : lchunk_large(solver);
   const SplitLargeLayout lay(lc, n, (int)sizeof(T));
   char *base = (char *)wk + WS_HEAD;
   T *Wm = (T *)(base + lay.wm), *de = (T *)(base + lay.de), *bw = (T *)(base + lay.bw);
@@ -627,8 +634,9 @@ hipError_t run_mixed(int L, int nm, int n, int m, double *a, double *w, void *wk
   const int iters = ((solver >> MIX_IT_SHIFT) & 3) ? ((solver >> MIX_IT_SHIFT) & 3) : MIX_DEFAULT_ITERS;
   const int sorted = (solver & FLAG_UNSORTED) ? 0 : 1;
   const bool wy = mixed_wy(n, solver);
-  const MixedLayout lay(L, n, wy);
-  const RefineLayout ly(L, n);
+  const int cb = solver & (3 << CHUNK_SHIFT);
+  const MixedLayout lay(L, n, wy, cb);
+  const RefineLayout ly(L, n, cb);
   char *base = (char *)wk + WS_HEAD;
   char *rb = base + lay.ref;
   float *a32 = (float *)(base + lay.a32), *w32 = (float *)(base + lay.w32);
@@ -645,10 +653,10 @@ hipError_t run_mixed(int L, int nm, int n, int m, double *a, double *w, void *wk
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   if (wy)
-    err = launch_split_large<float>(L, n, n, a32, w32, info32, SOLVER_IMTQL2 | FLAG_SPLIT,
+    err = launch_split_large<float>(L, n, n, a32, w32, info32, SOLVER_IMTQL2 | FLAG_SPLIT | cb,
                                     kopt_make(SOLVER_IMTQL2, false, 0), base + lay.ws32, s, base + lay.wyr);
   else
-    err = run<float>(L, n, n, n, a32, w32, base + lay.ws32, s, SOLVER_IMTQL2, info32);
+    err = run<float>(L, n, n, n, a32, w32, base + lay.ws32, s, SOLVER_IMTQL2 | cb, info32);
   if (err != hipSuccess) return err;
   for (long long c0 = 0; c0 < L; c0 += ly.lc) {
     const int cur = (int)((L - c0) < ly.lc ? (L - c0) : ly.lc);
@@ -828,6 +836,8 @@ std::vector<int> tune_candidates(int L, int n, int tb, int solver)
     c.push_back(FLAG_FUSED);
     if (n <= 64) {
       c.push_back(FLAG_SPLIT);
+      c.push_back(FLAG_SPLIT | FLAG_QL_LDS);
+      c.push_back(FLAG_SPLIT | (1 << CHUNK_SHIFT));
       return c;
     }
     c.push_back(FLAG_REG128);
@@ -1558,16 +1568,16 @@ hipError_t launch_params(int L, int n, int tb, int solver, int *p)
     p[1] = (nmat + g - 1) / g;
   };
   if (tb == 4 && use_split128(L, n, tb, solver)) {
-    grp(128, L < LCHUNK ? L : LCHUNK, 128);
+    grp(128, L < lchunk(solver) ? L : lchunk(solver), 128);
   } else if (use_split(L, n, solver)) {
-    grp(nb_of(n), L < LCHUNK ? L : LCHUNK, 64);
+    grp(nb_of(n), L < lchunk(solver) ? L : lchunk(solver), 64);
   } else if (base == SOLVER_JACOBI) {
     if (n > 64) return hipErrorNotSupported;
     grp(nb_of(n), L, 64);
   } else if (n <= 64) {
     grp(nb_of(n), L, 64);
   } else if (use_split_large(L, n, tb, solver)) {
-    const int lc = L < LCHUNK_LARGE ? L : LCHUNK_LARGE;
+    const int lc = L < lchunk_large(solver) ? L : lchunk_large(solver);
     const int tc = (solver >> TRI_CFG_SHIFT) & 3, gc = (solver >> GL_GRID_SHIFT) & 15;
     const bool sym = n <= 256 && !(solver & TRI_FULL_BIT);
     const int cfg = tc ? tc : (sym ? 1 : (n <= 256 ? 2 : 1));

@@ -36,6 +36,8 @@ FLAG_TRI_FULL = 0x4000000
 FLAG_REG128 = 0x8000
 # eigenpairs in the QL / Jacobi order instead of ascending (the reference's DO_SORT 0)
 FLAG_UNSORTED = 0x40000000
+# bits 18-19: the split paths' batch chunk divided by 2, 4 or 8 (smaller workspace, longer call)
+CHUNK_SHIFT = 18
 PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG,
          "split-lds": FLAG_SPLIT | FLAG_QL_LDS, "split-full": FLAG_SPLIT | FLAG_TRI_FULL,
          "split-reg128": FLAG_REG128, "split-reg128-fallback": FLAG_REG128 | FLAG_TINYLOG}

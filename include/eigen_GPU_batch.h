@@ -94,8 +94,12 @@ enum eigen_GPU_batch_solver {
  *          tridiagonalisation, lane-per-matrix QL, 64-row-block apply), the
  *          default there from 2048 matrices; 0x200 selects the large-order
  *          split path instead;
- *   bits 16-19 / 20-23  workgroup size / resident workgroup count of the
- *          workgroup kernels; bits 24-25 tridiagonalisation shape of the
+ *   bits 16-17 / 20-23  workgroup size / resident workgroup count of the
+ *          workgroup kernels; bits 18-19 (1..3) divide the split paths' batch
+ *          chunk (16384 matrices for n <= 128, 8192 beyond; 2048 for the
+ *          mixed solver's refinement) by 2, 4 or 8: the workspace shrinks by
+ *          that factor and the latency-bound QL recurrence runs once more per
+ *          chunk; bits 24-25 tridiagonalisation shape of the
  *          n > 64 split path; 0x4000000 (n <= 256) its full-storage form
  *          instead of the default lower-triangle form;
  *   0x40000000 eigenpairs in the order the QL iteration (or Jacobi) leaves
@@ -112,6 +116,7 @@ enum eigen_GPU_batch_solver {
 #define EIGEN_GPU_BATCH_REG128 0x8000
 #define EIGEN_GPU_BATCH_TRI_FULL 0x4000000
 #define EIGEN_GPU_BATCH_UNSORTED 0x40000000
+#define EIGEN_GPU_BATCH_CHUNK_SHIFT 18
 
 /* Workspace size in bytes for (L, nm, n, m, solver); 0 for invalid arguments. */
 size_t eigen_GPU_batch_bufsize(int L, int nm, int n, int m, int dtype_bytes, int solver);

@@ -243,3 +243,17 @@ def test_mixed_workspace_sizes():
     assert big >= fp32 + 8192 * n * n * 4 + 2048 * 6 * n * n * 8
     # plus the eigenvalues / status of the FP32 solve and the refinement's per-matrix results
     assert big <= fp32 + 8192 * n * n * 4 + 2048 * 6 * n * n * 8 + 8192 * (n + 1) * 8 + 2048 * (160 * n + 64) + (1 << 20)
+
+
+def test_chunk_bits_shrink_the_workspace():
+    """Bits 18-19 divide the split paths' chunk: the workspace shrinks by the
+    same factor (to the reference's lwork at n = 256 FP64 with a chunk of 2048,
+    eigen_GPU_batch.hpp:114-116 asks 4.31 GB there)."""
+    b = [eg.bufsize(16384, 64, 64, 64, 4, sh << eg.CHUNK_SHIFT) for sh in range(4)]
+    for s in range(1, 4):
+        assert abs(b[s] * (1 << s) - b[0]) <= (1 << s) * (1 << 16)
+    assert b[2] < 300e6 and b[3] < 150e6
+    d = [eg.bufsize(8192, 256, 256, 256, 8, sh << eg.CHUNK_SHIFT) for sh in range(4)]
+    assert d[1] < 8.7e9 and d[2] < 4.4e9
+    m = [eg.bufsize(8192, 256, 256, 256, 8, eg.MIXED | (sh << eg.CHUNK_SHIFT)) for sh in range(4)]
+    assert m[0] > m[1] > m[2] > m[3]

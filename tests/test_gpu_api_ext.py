@@ -96,3 +96,18 @@ def test_interleaved_layout(dtype, n, nm, L, path):
     assert np.array_equal(wi.t().cpu().numpy(), w)
     assert np.array_equal(zi[:, :, :n], z[:, :, :n])
     assert np.isnan(zi[:, :, n:]).all()                # padding rows carried through untouched
+
+
+@pytest.mark.parametrize("dtype,n,L,solver,shift", [(np.float32, 64, 5000, eg.IMTQL2, 3),
+                                                    (np.float32, 100, 4100, eg.IMTQL2, 2),
+                                                    (np.float64, 100, 1200, eg.IMTQL2, 3),
+                                                    (np.float64, 90, 600, eg.MIXED, 2)])
+def test_smaller_chunks_give_the_same_results(dtype, n, L, solver, shift):
+    """Bits 18-19 (the batch chunk divided by 2^shift): bitwise the same
+    eigenpairs as the default chunk, in a smaller workspace."""
+    print(f"[api] chunk/{1 << shift} n={n} L={L} {np.dtype(dtype).name} solver={solver}", flush=True)
+    a0 = torch.from_numpy(wl.sym_rand(L, n, n, dtype=dtype, seed=n + shift)).to(DEV)
+    path = "split" if (dtype == np.float64 and solver == eg.IMTQL2) else "auto"
+    z, w = run(a0, solver, path)
+    zc, wc = run(a0, solver | (shift << eg.CHUNK_SHIFT), path)
+    assert np.array_equal(w, wc) and np.array_equal(z, zc)
