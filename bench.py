@@ -152,7 +152,7 @@ def time_phases(a0, steps, warmup, path="split"):
     """Per-phase HIP-event times (ms) of the split paths: the phases are issued
     as separate library calls on one stream with events between them (n <= 64:
     tri = eig_tri_kernel, ql = eig_ql_reg_kernel, apply = eig_apply_kernel plus
-    the overflow-fallback launch; larger n: eig_tri_large_kernel,
+    the overflow-fallback launch; larger n: eig_tri_reg_kernel (n <= 256),
     eig_ql_lane_kernel, eig_apply_large_kernel plus the fallback launch, whose
     workgroups exit at once when no matrix overflowed).  Returns {phase: mean ms}."""
     import numpy as np
@@ -528,7 +528,7 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
             "roofline": {"bound": "valu", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                         "kernel": "eig_tri_large_kernel + eig_ql_lane_kernel + eig_apply_large_kernel "
+                         "kernel": "eig_tri_reg_kernel + eig_ql_lane_kernel + eig_apply_large_kernel "
                                    "(+ overflow-fallback launch)",
                          "kernel_ms": kms2, "phase_ms": ph2,
                          "note": "(10/3)n^3 flops per matrix against the FP64 vector peak"},

@@ -233,4 +233,59 @@ __device__ __forceinline__ void st8(double *p, const double o[8])
   for (int q = 0; q < 4; ++q) *(double2 *)(p + 2 * q) = make_double2(o[2 * q], o[2 * q + 1]);
 }
 
+// One step of a butterfly reduce-scatter: a lane and its partner under Mv
+// hold 2K partial values for the same 2K indices and opposite `side`s; the lane
+// keeps half `side` (indices [K side, K side + K)) with the partner's partials
+// added, in v[0..K).
+template <int K, typename T, typename Mv>
+__device__ __forceinline__ void reduce_scatter_step(T *v, bool side, Mv mv)
+{
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const T keep = side ? v[i + K] : v[i];
+    const T send = side ? v[i] : v[i + K];
+    v[i] = keep + lane_move(send, mv);
+  }
+}
+
+// Pair sums over the gfx950 lane swaps (VALU, no LDS): v_permlane16_swap
+// exchanges the odd 16-lane row of each pair (rows 2k, 2k+1) of its first
+// operand with the even row of its second; v_permlane32_swap the upper 32
+// lanes of the first with the lower 32 of the second.  With (a, b) in every
+// lane, swap_sum<W>(a, b) returns on the low side (even row / lanes 0-31)
+// a + the partner lane's a, and on the high side b + the partner's b, where
+// the partner is lane ^ W: one butterfly reduce-scatter step of one pair.
+template <int W>
+__device__ __forceinline__ unsigned swap_u32(unsigned a, unsigned b, unsigned &bo)
+{
+  static_assert(W == 16 || W == 32, "row pairs or wave halves");
+  if constexpr (W == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    bo = r[1];
+    return r[0];
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    bo = r[1];
+    return r[0];
+  }
+}
+template <int W>
+__device__ __forceinline__ float swap_sum(float a, float b)
+{
+  unsigned bo;
+  const unsigned ao = swap_u32<W>(__float_as_uint(a), __float_as_uint(b), bo);
+  return __uint_as_float(ao) + __uint_as_float(bo);
+}
+template <int W>
+__device__ __forceinline__ double swap_sum(double a, double b)
+{
+  const unsigned long long ab = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long bb = (unsigned long long)__double_as_longlong(b);
+  unsigned blo, bhi;
+  const unsigned alo = swap_u32<W>((unsigned)ab, (unsigned)bb, blo);
+  const unsigned ahi = swap_u32<W>((unsigned)(ab >> 32), (unsigned)(bb >> 32), bhi);
+  return __longlong_as_double((long long)(((unsigned long long)ahi << 32) | alo)) +
+         __longlong_as_double((long long)(((unsigned long long)bhi << 32) | blo));
+}
+
 } // namespace eigmi

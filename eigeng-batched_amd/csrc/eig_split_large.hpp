@@ -247,21 +247,6 @@ __device__ void blocked_tridiag(T *__restrict__ A, int nm, int n, TriBlkLds<T, N
   __syncthreads();
 }
 
-// One step of a butterfly reduce-scatter: a lane and its partner under Mv
-// hold 2K partial values for the same 2K indices and opposite `side`s; the lane
-// keeps half `side` (indices [K side, K side + K)) with the partner's partials
-// added, in v[0..K).
-template <int K, typename T, typename Mv>
-__device__ __forceinline__ void reduce_scatter_step(T *v, bool side, Mv mv)
-{
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const T keep = side ? v[i + K] : v[i];
-    const T send = side ? v[i] : v[i + K];
-    v[i] = keep + lane_move(send, mv);
-  }
-}
-
 // Symmetric-storage form of blocked_tridiag (the default for n <= 256): the
 // live matrix is the lower triangle with the diagonal, loaded as the transpose
 // of the input's upper triangle.  The SYMV reads each lower element once per

@@ -17,6 +17,7 @@
 #include "eig_split.hpp"
 #include "eig_uneven.hpp"
 #include "eig_split_large.hpp"
+#include "eig_tri_reg.hpp"
 #include "eig_check.hpp"
 #include "eig_symdmt.hpp"
 #include "eig_gemm.hpp"
@@ -312,9 +313,10 @@ hipError_t launch_jacobi(int L, int nm, int n, T *a, T *w, int *info, int kop, h
 // defaults.  Fewer, wider workgroups keep the matrices being worked on (and
 // their accumulators) inside the on-chip caches.
 constexpr int GL_NT_SHIFT = 16, GL_GRID_SHIFT = 20;
-// large split path, tridiagonalisation shape (bits 24-25): 1: 256 threads,
-// 2: 512 threads in 2 column groups, 3: 1024 threads in 4 column groups
-// (2 for n > 256); 0 picks the default
+// large split path, tridiagonalisation shape (bits 24-25): 0: for n <= 256 the
+// register-resident kernel (eig_tri_reg.hpp), beyond the default below; the
+// workspace-resident kernels: 1: 256 threads, 2: 512 threads in 2 column
+// groups, 3: 1024 threads in 4 column groups (2 for n > 256)
 constexpr int TRI_CFG_SHIFT = 24;
 // bit 26: for n <= 256, the full-storage tridiagonalisation instead of the
 // default symmetric (lower-triangle) one
@@ -455,7 +457,16 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
       const int cfg = tc ? tc : (sym ? 1 : (n <= 256 ? 2 : 1));
       unsigned grid = tg;
       if (gc) grid = (unsigned)(128 * gc) < grid ? (unsigned)(128 * gc) : grid;
-      if (sym) {
+      if (sym && tc == 0) {
+        // register-resident: 12 waves own the <= 36 tiles of n <= 256 (one
+        // workgroup per CU), 4 waves the <= 10 tiles of n <= 128
+        if (n <= 128)
+          hipLaunchKernelGGL((eig_tri_reg_kernel<T, 4, 3, 3>), dim3(grid), dim3(256), 0, s, cur, nm, n, ac, Wm, de,
+                             lc, bw);
+        else
+          hipLaunchKernelGGL((eig_tri_reg_kernel<T, 12, 3, 3>), dim3(grid), dim3(768), 0, s, cur, nm, n, ac, Wm, de,
+                             lc, bw);
+      } else if (sym) {
         if (cfg == 3)
           hipLaunchKernelGGL((eig_tri_large_kernel<T, 1024, TriNbp<T>::value, 256, 4, true>), dim3(grid), dim3(1024), 0, s, cur,
                              nm, n, ac, Wm, de, lc, bw);
@@ -852,6 +863,7 @@ std::vector<int> tune_candidates(int L, int n, int tb, int solver)
   // tridiagonalisation shapes: symmetric form 256 / 512 threads and the full
   // form with 512 threads (n <= 256); 256 / 1024 threads beyond
   const int cfs[2] = {1, n <= 256 ? 2 : 3};
+  if (n <= 256) c.push_back(FLAG_SPLIT);
   for (int cf : cfs) {
     c.push_back(FLAG_SPLIT | (cf << TRI_CFG_SHIFT));
     if (L > 512) c.push_back(FLAG_SPLIT | (cf << TRI_CFG_SHIFT) | (4 << GL_GRID_SHIFT));

@@ -33,6 +33,10 @@ GEMM_F64, GEMM_F32, GEMM_F16, GEMM_BF16, GEMM_F16X3 = 0, 1, 2, 3, 4
 FLAG_FUSED, FLAG_SPLIT, FLAG_TINYLOG = 0x100, 0x200, 0x400
 FLAG_QL_LDS = 0x800
 FLAG_TRI_FULL = 0x4000000
+# bits 24-25: the large split path's tridiagonalisation kernel; 0 (default) the
+# register-resident one for n <= 256, 1 the workspace-resident lower-triangle one
+TRI_CFG_SHIFT = 24
+FLAG_TRI_WS = 1 << TRI_CFG_SHIFT
 FLAG_REG128 = 0x8000
 # eigenpairs in the QL / Jacobi order instead of ascending (the reference's DO_SORT 0)
 FLAG_UNSORTED = 0x40000000
@@ -40,6 +44,7 @@ FLAG_UNSORTED = 0x40000000
 CHUNK_SHIFT = 18
 PATHS = {"auto": 0, "fused": FLAG_FUSED, "split": FLAG_SPLIT, "split-fallback": FLAG_SPLIT | FLAG_TINYLOG,
          "split-lds": FLAG_SPLIT | FLAG_QL_LDS, "split-full": FLAG_SPLIT | FLAG_TRI_FULL,
+         "split-ws": FLAG_SPLIT | FLAG_TRI_WS,
          "split-reg128": FLAG_REG128, "split-reg128-fallback": FLAG_REG128 | FLAG_TINYLOG}
 # split-path phases (a call with a subset runs only those kernels; the phases
 # hand over through the workspace, so tri -> ql -> apply calls on one stream
@@ -182,7 +187,9 @@ def eigen_batch(a, w=None, m: int | None = None, solver: int = IMTQL2, info=None
     rotation log shrunk so that every matrix takes the overflow fallback) or
     "split-lds" (split with the QL recurrence's d, e kept in LDS rather than in
     registers) or, for 64 < n <= 256, "split-full" (split with the
-    tridiagonalisation on the full matrix instead of its lower triangle).
+    tridiagonalisation on the full matrix instead of its lower triangle) or
+    "split-ws" (split with the workspace-resident lower-triangle
+    tridiagonalisation instead of the register-resident one).
     phases (split path only): iterable of "tri", "ql", "apply" to run.
     m < n: all n eigenvalues, eigenvectors of the first m only (columns m..n-1
     of each matrix are left unspecified).  unsorted: the QL / Jacobi order.

@@ -152,7 +152,8 @@ def test_large_n_vs_oracle(n, dtype):
 LARGE_SPLIT_SIZES = [65, 96, 129, 160, 200, 255, 256, 257, 300, 384, 512]
 
 
-@pytest.mark.parametrize("path", ["split", "split-fallback", "split-full", "split-reg128", "split-reg128-fallback"])
+@pytest.mark.parametrize("path", ["split", "split-fallback", "split-full", "split-ws", "split-reg128",
+                                  "split-reg128-fallback"])
 @pytest.mark.parametrize("solver", [eg.IMTQL2, eg.TQL2])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", LARGE_SPLIT_SIZES)
@@ -160,10 +161,11 @@ def test_large_split_vs_oracle(n, dtype, solver, path):
     """Orders beyond 64 through the split path (workgroup tridiagonalisation,
     lane-per-matrix QL with rotation log, row-block apply), through its
     overflow fallback (log capacity forced to 1), for n <= 256 with the
-    full-storage tridiagonalisation instead of the lower-triangle one, and, for
+    full-storage and the workspace-resident lower-triangle tridiagonalisations
+    instead of the register-resident one, and, for
     FP32 n <= 128, through the register split path (two-wave
     tridiagonalisation, 64-row-block apply) and its fallback."""
-    if path == "split-full" and n > 256:
+    if path in ("split-full", "split-ws") and n > 256:
         pytest.skip("the full-storage form is the only one beyond n = 256")
     if path.startswith("split-reg128") and (dtype != np.float32 or n > 128):
         pytest.skip("the register split path serves FP32 64 < n <= 128")
@@ -180,7 +182,7 @@ def test_large_split_vs_oracle(n, dtype, solver, path):
     check_batch(np.nan_to_num(a0), z, w, dtype, wo, zo)
 
 
-@pytest.mark.parametrize("path", ["split", "split-full", "fused", "split-reg128"])
+@pytest.mark.parametrize("path", ["split", "split-full", "split-ws", "fused", "split-reg128"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [65, 128, 129, 256, 300])
 def test_large_upper_triangle_only(n, dtype, path):
@@ -199,6 +201,31 @@ def test_large_upper_triangle_only(n, dtype, path):
     assert i0.max() == 0 and i1.max() == 0
     np.testing.assert_array_equal(w1, w0)
     np.testing.assert_array_equal(z1, z0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [97, 200, 256])
+def test_register_tridiagonalisation_many_matrices_per_workgroup(n, dtype):
+    """The register-resident tridiagonalisation with a grid of 128 workgroups
+    (grid code 1 in bits 20-23), so each workgroup reduces several matrices in
+    turn; eigenpairs against the oracle on a sample and the GPU checker on all,
+    and eigenvalues against the workspace-resident kernel."""
+    print(f"[parity] register tri, persistent grid n={n} dtype={np.dtype(dtype).name}", flush=True)
+    L = 300
+    a0 = wl.sym_rand(L, n, n, dtype=dtype, seed=7300 + n)
+    t = torch.from_numpy(a0).to(DEV)
+    z, w, info = eg.eigen_batch(t.clone(), solver=eg.IMTQL2 | eg.FLAG_SPLIT | (1 << 20))
+    z2, w2, info2 = eg.eigen_batch(t.clone(), path="split-ws")
+    torch.cuda.synchronize()
+    assert int(info.abs().max()) == 0 and int(info2.abs().max()) == 0
+    out = eg.check(t, w, z).cpu().numpy()
+    assert int(out[:, 3].max()) == 0, f"checker flags on {int((out[:, 3] != 0).sum())} matrices"
+    wn, w2n = w.cpu().numpy(), w2.cpu().numpy()
+    scale = np.abs(wn).max(axis=1, keepdims=True)
+    assert np.max(np.abs(wn - w2n) / scale) <= ref_tol(n, dtype)
+    idx = [0, 1, 150, L - 1]
+    zo, wo, _ = oracle.eig_batch(np.ascontiguousarray(a0[idx]), oracle.IMTQL2)
+    check_batch(a0[idx], z.cpu().numpy()[idx], wn[idx], dtype, wo, zo)
 
 
 @pytest.mark.parametrize("dtype", [np.float64])
