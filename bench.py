@@ -500,7 +500,7 @@ def _traffic(name):
         return json.load(f).get("bytes_per_launch")
 
 
-def bench_n256(steps, warmup, vendor=True, tuned=True):
+def bench_n256(steps, warmup, vendor=True, tuned=True, mixed=True):
     """L = 8192, n = 256, FP64 line: whole-call time, per-phase times, roofline,
     accuracy of every matrix."""
     import torch
@@ -513,8 +513,8 @@ def bench_n256(steps, warmup, vendor=True, tuned=True):
     torch.cuda.empty_cache()
     tuned2 = tuned_line(q0, ms2, steps, warmup) if tuned else None
     vend = vendor_baseline(q0[:512].contiguous(), reps=1) if vendor else None
-    mixed = bench_mixed(q0, ms2, steps, warmup)
-    chunks2 = chunk_tradeoff(q0, (0, 1), steps=steps, warmup=warmup)
+    mixed = bench_mixed(q0, ms2, steps, warmup) if mixed else None
+    chunks2 = chunk_tradeoff(q0, (0, 1), steps=steps, warmup=warmup) if mixed else None
     ph2, a2, w2, _ = time_phases(q0, 2, 1)
     del q0, a2, w2
     torch.cuda.empty_cache()
@@ -858,6 +858,8 @@ def main():
     ap.add_argument("--ws-total-steps", type=int, default=200, help="warm-start line: stream length")
     ap.add_argument("--ws-eval-steps", type=int, nargs="+", default=[50, 100, 150, 199],
                     help="warm-start line: evaluation steps")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="n256 line without the mixed-precision and chunk sub-lines (for profiling)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="print only the n=256 FP64 mixed-precision line (and the plain time)")
     ap.add_argument("--uneven-count", type=int, default=100000)
@@ -887,7 +889,7 @@ def main():
         return
     if args.n256_only:
         print(json.dumps({"n256": bench_n256(args.steps, args.warmup, vendor=not args.no_vendor,
-                                             tuned=not args.no_tuned)}))
+                                             tuned=not args.no_tuned, mixed=not args.no_mixed)}))
         return
     if args.letkf_step_only:
         print(json.dumps({"letkf_step": bench_letkf_step(args.steps, args.warmup)}))
