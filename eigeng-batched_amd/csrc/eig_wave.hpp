@@ -201,17 +201,22 @@ __device__ __forceinline__ void wave_front(const T *__restrict__ A, int nm, int 
     vset<NL, NW>(ev, kl, c - 1, anorm);
     vset<NL, NW>(bv, kl, c, beta);
     gsync<NW>();
-    // p = beta * A(0:c-1, 0:c-1) u  (row k of the product)
-    T p = T(0);
+    // p = beta * A(0:c-1, 0:c-1) u  (row k of the product), in two partial
+    // sums so that the FMA chain is half as long
+    T p = T(0), p1 = T(0);
 #pragma unroll
     for (int b = 0; b < NBLK; ++b) {
       if (b < nbc) {
         T u8[8];
         ld8(&S.U[off + 8 * b], u8);
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) p = fma_(R[8 * b + jj], u8[jj], p);
+        for (int jj = 0; jj < 8; jj += 2) {
+          p = fma_(R[8 * b + jj], u8[jj], p);
+          p1 = fma_(R[8 * b + jj + 1], u8[jj + 1], p1);
+        }
       }
     }
+    p += p1;
     p = (k < c) ? p * beta : T(0);
     const T alpha = gsum<NB>(p * uk, S.red, 1, wv, kl) * (beta * T(0.5));
     const T vk = fma_(alpha, uk, p);
@@ -244,17 +249,20 @@ __device__ __forceinline__ void wave_front(const T *__restrict__ A, int nm, int 
     const int off = uoff(c);
     const int nbc = (c + 7) >> 3;
     const T beta = vget<NL, NW>(bv, c);
-    T t = T(0);
+    T t = T(0), t1 = T(0);
 #pragma unroll
     for (int b = 0; b < NBLK; ++b) {
       if (b < nbc) {
         T u8[8];
         ld8(&S.U[off + 8 * b], u8);
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) t = fma_(R[8 * b + jj], u8[jj], t);
+        for (int jj = 0; jj < 8; jj += 2) {
+          t = fma_(R[8 * b + jj], u8[jj], t);
+          t1 = fma_(R[8 * b + jj + 1], u8[jj + 1], t1);
+        }
       }
     }
-    t *= beta;
+    t = (t + t1) * beta;
 #pragma unroll
     for (int b = 0; b < NBLK; ++b) {
       if (b < nbc) {
