@@ -739,7 +739,7 @@ def bench_n128(steps, warmup):
             "roofline": {"bound": "valu", "achieved": L2 * f2 / (kms2 * 1e-3) / 1e12,
                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": L2 * f2 / (kms2 * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
-                         "kernel": "eig_tri_kernel<float,128> + eig_ql_lane_kernel + eig_apply_kernel<float,128> "
+                         "kernel": "eig_tri_kernel<float,128> + eig_ql_reg_kernel<float,128> + eig_apply_kernel<float,128> "
                                    "(+ overflow-fallback launch)",
                          "kernel_ms": kms2, "phase_ms": ph2,
                          "note": "(10/3)n^3 flops per matrix against the FP32 vector peak"},

@@ -336,6 +336,52 @@ __device__ __forceinline__ bool givens_sel(double f, double g, double &r, double
   return ok;
 }
 
+// Bit mask over the off-diagonal indices 0 .. NB-2 of the register QL kernel:
+// one 64-bit word up to NB = 64, two beyond.
+template <int NB>
+struct QlMask {
+  static constexpr int W = NB <= 64 ? 1 : 2;
+  uint64_t w[W];
+};
+// bits l, l+1, ... set
+template <int NB>
+__device__ __forceinline__ QlMask<NB> qm_from(int l)
+{
+  QlMask<NB> m;
+  m.w[0] = (l < 64) ? (~0ull << (l < 0 ? 0 : l)) : 0ull;
+  if constexpr (QlMask<NB>::W == 2) m.w[1] = (l <= 64) ? ~0ull : ((l < 128) ? (~0ull << (l - 64)) : 0ull);
+  return m;
+}
+template <int NB>
+__device__ __forceinline__ QlMask<NB> qm_and(const QlMask<NB> &a, const QlMask<NB> &b)
+{
+  QlMask<NB> m;
+#pragma unroll
+  for (int q = 0; q < QlMask<NB>::W; ++q) m.w[q] = a.w[q] & b.w[q];
+  return m;
+}
+template <int NB>
+__device__ __forceinline__ QlMask<NB> qm_andnot(const QlMask<NB> &a, const QlMask<NB> &b)   // a & ~b
+{
+  QlMask<NB> m;
+#pragma unroll
+  for (int q = 0; q < QlMask<NB>::W; ++q) m.w[q] = a.w[q] & ~b.w[q];
+  return m;
+}
+template <int NB>
+__device__ __forceinline__ bool qm_zero(const QlMask<NB> &a)
+{
+  if constexpr (QlMask<NB>::W == 2) return (a.w[0] | a.w[1]) == 0ull;
+  else return a.w[0] == 0ull;
+}
+// lowest set bit (the mask is not empty)
+template <int NB>
+__device__ __forceinline__ int qm_ctz(const QlMask<NB> &a)
+{
+  if constexpr (QlMask<NB>::W == 2) return a.w[0] ? __builtin_ctzll(a.w[0]) : 64 + __builtin_ctzll(a.w[1]);
+  else return __builtin_ctzll(a.w[0]);
+}
+
 // IMTQL2 recurrence, one matrix per lane, with d and e in registers.
 //
 // All lanes of the wave walk the same register index at the same time: each
@@ -353,7 +399,7 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
     int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int opts = 0)
 {
   typedef typename vec2<T>::type T2;
-  static_assert(NB <= 64, "one 64-bit mask per lane");
+  static_assert(NB <= 128, "at most two 64-bit mask words per lane");
   const int lane = threadIdx.x & 63;
   const long long mat = (long long)blockIdx.x * 64 + lane;
   const bool live = mat < L;
@@ -367,8 +413,8 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
   T2 *__restrict__ lg = rotlog + (size_t)mt * rot_cap;
   int *__restrict__ sw = swlog + (size_t)mt * sw_cap;
   const T tol = eps_of<T>() * T(sizeof(T) == 8 ? 512 : 16);
-  // bits n-1 .. 63 always set: e(n-1) = 0 ends every window at n-1 at the latest
-  const uint64_t top = (n - 1 >= 64) ? 0ull : (~0ull << (n - 1));
+  // bits from n-1 on always set: e(n-1) = 0 ends every window at n-1 at the latest
+  const QlMask<NB> top = qm_from<NB>(n - 1);
   int l = 0, itr = 0, st = 0, nrot = 0, nsw = 0;
   bool run = live && n > 1;
 
@@ -377,26 +423,26 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
     // ---- window of this round ----------------------------------------------
     // negligible off-diagonals below every running lane's l are never looked
     // at again, so 8-blocks entirely below lmin are skipped
-    uint64_t small = top;
+    QlMask<NB> small = top;
 #pragma unroll
     for (int b = 0; b < NB / 8; ++b) {
       if (8 * b + 8 > lmin) {
 #pragma unroll
         for (int i = 8 * b; i < 8 * b + 8 && i < NB - 1; ++i) {
           const bool sm = !(abs_(e[i]) > (abs_(d[i]) + abs_(d[i + 1])) * tol);  // NaN counts as negligible
-          small |= (sm && i < n - 1) ? (1ull << i) : 0ull;
+          small.w[i >> 6] |= (sm && i < n - 1) ? (1ull << (i & 63)) : 0ull;
         }
       }
     }
     if (run && itr > QL_MAX_SWEEP) { st = l + 1; run = false; }
-    const uint64_t cand = ~small & (~0ull << l);
-    if (run && cand == 0ull) run = false;        // every eigenvalue converged
+    const QlMask<NB> cand = qm_andnot(qm_from<NB>(l), small);
+    if (run && qm_zero(cand)) run = false;       // every eigenvalue converged
     int m = 0;
     if (run) {
-      const int nl = __builtin_ctzll(cand);
+      const int nl = qm_ctz(cand);
       if (nl != l) itr = 0;
       l = nl;
-      m = __builtin_ctzll(small & (~0ull << l));
+      m = qm_ctz(qm_and(small, qm_from<NB>(l)));
       if (nsw >= sw_cap || nrot + (m - l) > rot_cap) { st = ST_OVERFLOW; run = false; }
     }
     const bool sweep = run;

@@ -277,10 +277,10 @@ hipError_t launch_split(int L, int nm, int n, T *a, T *w, int *info, int solver,
     if (ph & FLAG_PH_TRI)
       hipLaunchKernelGGL((eig_tri_kernel<T, NB>), dim3(gb), dim3(64 * NW), 0, s, cur, nm, n, ac, q, de, lc);
     if (ph & FLAG_PH_QL) {
-      // registers hold d and e (2 NB values per lane): FP32 up to NB = 64, FP64 up to 32
-      const bool reg = (solver & 0xff) == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS) && NB * sizeof(T) <= 256;
+      // registers hold d and e (2 NB values per lane): FP32 up to NB = 128, FP64 up to 64
+      const bool reg = (solver & 0xff) == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS) && NB * sizeof(T) <= 512;
       if (reg)
-        hipLaunchKernelGGL((eig_ql_reg_kernel<T, (NB * sizeof(T) <= 256 ? NB : 8)>), dim3(lb), dim3(64), 0, s,
+        hipLaunchKernelGGL((eig_ql_reg_kernel<T, (NB * sizeof(T) <= 512 ? NB : 8)>), dim3(lb), dim3(64), 0, s,
                            cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, kop);
       else
         hipLaunchKernelGGL((eig_ql_lane_kernel<T, NB>), dim3(lb), dim3(64), 0, s, cur, n, de, lc, rot, rcap,
