@@ -392,23 +392,42 @@ __device__ __forceinline__ int qm_ctz(const QlMask<NB> &a)
 // sweep limit and the logs are those of eig_ql_lane_kernel's IMTQL2 branch
 // (imtql2_tiled.hpp:43-164), and the final ordering is the reference's
 // selection sort (tql2_tiled.hpp:149-176), also unrolled over registers.
-template <typename T, int NB>
+// LIST: the uneven-batch form (u): lane slot idx holds list entry idx of the
+// chunk, with its own order (d / e at rows i / nmx + i of de), as in
+// eig_ql_lane_kernel.
+template <typename T, int NB, bool LIST = false>
 __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
     int L, int n, const T *__restrict__ de, int ld, typename vec2<T>::type *__restrict__ rotlog,
     int rot_cap, int *__restrict__ swlog, int sw_cap, int *__restrict__ nsw_out, int *__restrict__ status,
-    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int opts = 0)
+    int *__restrict__ slot, T *__restrict__ w, int *__restrict__ info, int opts = 0, UvSplitArgs u = UvSplitArgs{})
 {
   typedef typename vec2<T>::type T2;
   static_assert(NB <= 128, "at most two 64-bit mask words per lane");
   const int lane = threadIdx.x & 63;
-  const long long mat = (long long)blockIdx.x * 64 + lane;
-  const bool live = mat < L;
+  const long long mat = (long long)blockIdx.x * 64 + lane;   // the slot (LIST) or the matrix
+  bool live = mat < L;
+  long long gmat = mat;                                        // the matrix (info)
+  int eoff = n;                                                // row of e(0) in de
+  T *__restrict__ wout;
+  int *__restrict__ sout;
+  if constexpr (LIST) {
+    int mi = 0;
+    live = live && uv_split_entry(u, mat, mi);
+    gmat = mi;
+    n = live ? u.nv[mi] : 1;                                   // this lane's order
+    eoff = u.nmx;
+    wout = w + (live ? u.woff[mi] : 0);
+    sout = slot + (size_t)(live ? mat : 0) * u.nmx;
+  } else {
+    wout = w + (size_t)(live ? mat : 0) * n;
+    sout = slot + (size_t)(live ? mat : 0) * n;
+  }
   const long long mt = live ? mat : 0;
   T d[NB], e[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     d[i] = (live && i < n) ? de[(size_t)i * ld + mt] : T(0);
-    e[i] = (live && i < n) ? de[(size_t)(n + i) * ld + mt] : T(0);
+    e[i] = (live && i < n) ? de[(size_t)(eoff + i) * ld + mt] : T(0);
   }
   T2 *__restrict__ lg = rotlog + (size_t)mt * rot_cap;
   int *__restrict__ sw = swlog + (size_t)mt * sw_cap;
@@ -532,11 +551,11 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
   status[mat] = st;
   nsw_out[mat] = nsw;
   if (st == ST_OVERFLOW) return;                 // solved again by the fused kernel
-  if (info) info[mat] = st;
+  if (info) info[gmat] = st;
   if (st != 0) {
 #pragma unroll
     for (int i = 0; i < NB; ++i)
-      if (i < n) w[(size_t)mat * n + i] = d[i];
+      if (i < n) wout[i] = d[i];
     return;
   }
   // selection sort with the reference's rule; e now holds the positions
@@ -570,8 +589,8 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     if (i < n) {
-      w[(size_t)mat * n + i] = d[i];
-      slot[(size_t)mat * n + (int)e[i]] = i < mcols ? i : -1;
+      wout[i] = d[i];
+      sout[(int)e[i]] = i < mcols ? i : -1;
     }
   }
 }

@@ -1069,8 +1069,12 @@ void run_uneven_split(int L, int nmx, int *cc, const int *list, const int *nv, c
       // the replay), Q in the Wm slots
       hipLaunchKernelGGL((eig_tri_kernel<T, 128, true>), dim3(cap), dim3(128), 0, s, cap, 0, 0, (const T *)a, Wm,
                          de, cap, u);
-      hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0,
-                         s, cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+      if (sv == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS))
+        hipLaunchKernelGGL((eig_ql_reg_kernel<T, 128, true>), dim3((cap + 63) / 64), dim3(64), 0, s, cap, 0,
+                           (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+      else
+        hipLaunchKernelGGL((eig_ql_lane_kernel<T, QMAX, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0,
+                           s, cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
       hipLaunchKernelGGL((eig_apply_kernel<T, 128, true>), dim3((unsigned)cap * 2), dim3(64), 0, s, cap, 0, 0, a,
                          (const T *)Wm, (const T2 *)rot, rcap, (const int *)sw, scap, (const int *)nsw,
                          (const int *)st, (const int *)slot, u);
@@ -1119,8 +1123,12 @@ void run_uneven_split64(int L, int *cc, const int *list, const int *nv, const lo
     const UvSplitArgs u{cc, list, nv, ao, wo, (int)c0, cap, min_count, nmx};
     hipLaunchKernelGGL((eig_tri_kernel<T, 64, true>), dim3(cap), dim3(64), 0, s, cap, 0, 0, (const T *)a, Wm, de,
                        cap, u);
-    hipLaunchKernelGGL((eig_ql_lane_kernel<T, 64, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0, s,
-                       cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+    if (sv == SOLVER_IMTQL2 && !(solver & FLAG_QL_LDS))
+      hipLaunchKernelGGL((eig_ql_reg_kernel<T, 64, true>), dim3((cap + 63) / 64), dim3(64), 0, s, cap, 0,
+                         (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
+    else
+      hipLaunchKernelGGL((eig_ql_lane_kernel<T, 64, NLANE, true>), dim3((cap + NLANE - 1) / NLANE), dim3(64), 0, s,
+                         cap, 0, (const T *)de, cap, rot, rcap, sw, scap, nsw, st, slot, w, info, sv, u);
     hipLaunchKernelGGL((eig_apply_kernel<T, 64, true>), dim3(cap), dim3(64), 0, s, cap, 0, 0, a, (const T *)Wm,
                        (const T2 *)rot, rcap, (const int *)sw, scap, (const int *)nsw, (const int *)st,
                        (const int *)slot, u);
