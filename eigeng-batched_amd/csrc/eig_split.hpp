@@ -311,29 +311,24 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
   }
 }
 
-// Branch-free Givens pair for the lockstep kernel: same values as givens()
-// when (f, g) != 0; otherwise r = 0 and c, s keep their inputs.
-__device__ __forceinline__ bool givens_sel(float f, float g, float &r, float &c, float &s)
+// Givens pair of the register QL step for ss = f^2 + g^2 > 0: r = sqrt(ss),
+// c = g/r, s = f/r from one refined reciprocal square root; in FP32 (c, s)
+// renormalised once with the fma-exact residual c^2 + s^2 - 1.
+__device__ __forceinline__ void givens_nz(float f, float g, float ss, float &r, float &c, float &s)
 {
-  const float ss = __builtin_fmaf(f, f, g * g);
-  const bool ok = ss != 0.0f;
-  const float ri = rsqrt_nr(ok ? ss : 1.0f);
+  const float ri = rsqrt_nr(ss);
   const float cn = g * ri, sn = f * ri;
   const float h = -0.5f * __builtin_fmaf(cn, cn, __builtin_fmaf(sn, sn, -1.0f));
-  r = ok ? ss * ri : 0.0f;
-  c = ok ? __builtin_fmaf(cn, h, cn) : c;
-  s = ok ? __builtin_fmaf(sn, h, sn) : s;
-  return ok;
+  r = ss * ri;
+  c = __builtin_fmaf(cn, h, cn);
+  s = __builtin_fmaf(sn, h, sn);
 }
-__device__ __forceinline__ bool givens_sel(double f, double g, double &r, double &c, double &s)
+__device__ __forceinline__ void givens_nz(double f, double g, double ss, double &r, double &c, double &s)
 {
-  const double ss = __builtin_fma(f, f, g * g);
-  const bool ok = ss != 0.0;
-  const double ri = rsqrt_nr(ok ? ss : 1.0);
-  r = ok ? ss * ri : 0.0;
-  c = ok ? g * ri : c;
-  s = ok ? f * ri : s;
-  return ok;
+  const double ri = rsqrt_nr(ss);
+  r = ss * ri;
+  c = g * ri;
+  s = f * ri;
 }
 
 // Bit mask over the off-diagonal indices 0 .. NB-2 of the register QL kernel:
@@ -507,40 +502,58 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
     const unsigned span = sweep ? (unsigned)(hi - lo) : 0u;   // idle lanes: i - lo < 0 fails
     T s = T(1), c = T(1), delta = T(0);
     int ilo = l;
+    T2 *lp = lg + nrot;   // next free slot of this lane's rotation log
 #pragma unroll
     for (int b = NB / 8 - 1; b >= 0; --b) {
       if (8 * b <= hi_w && 8 * b + 7 >= lo_w) {
 #pragma unroll
         for (int i = 8 * b + 7; i >= 8 * b; --i) {
           if (i <= NB - 2) {
-            const bool act = (unsigned)(i - lo) <= span;
-            const T ei = e[i];
-            const T f = s * ei, bb = c * ei;
-            T rn, cn = c, sn = s;
-            const bool ok = givens_sel(f, r, rn, cn, sn);
-            const bool go = act && ok, brk = act && !ok;
-            const T dix = d[i + 1] - delta;
-            const T q = (d[i] - dix) * sn + T(2) * cn * bb;
-            const T dx1 = fma_(sn, q, dix);
-            if (go) lg[nrot] = T2{cn, sn};
-            nrot += go ? 1 : 0;
-            // e(i+1) = r (e(m) ends at 0; 0 after an underflow break), d(i+1) = dx1
-            e[i + 1] = act ? ((i == hi || brk) ? T(0) : rn) : e[i + 1];
-            d[i + 1] = act ? (go ? dx1 : dix) : d[i + 1];
-            delta = go ? dx1 - dix : delta;
-            r = go ? cn * q - bb : r;
-            c = go ? cn : c;
-            s = go ? sn : s;
-            // end of the window: d(l) -= delta, e(l) = r
-            const bool fin = go && (i == lo);
-            d[i] = fin ? d[i] - delta : d[i];
-            e[i] = fin ? r : e[i];
-            ilo = brk ? i + 1 : ilo;
-            lo = brk ? NB + 8 : lo;    // a break ends this lane's sweep
+            // only the lanes whose window holds i run the step (the exec
+            // mask), so the values carried along the chase need no selects.
+            // The step is EISPACK's: s = f/r, c = g/r, g = d(i+1) - p,
+            // r = (d(i) - g) s + 2 c b, p = s r, d(i+1) = g + p, g = c r - b;
+            // e(i+1) = r except e(m) = 0; at the end of the window d(l) -= p,
+            // e(l) = g.
+            if ((unsigned)(i - lo) <= span) {
+              const T ei = e[i];
+              const T f = s * ei, bb = c * ei;
+              const T ss = fma_(f, f, r * r);
+              const bool brk = ss == T(0);
+              const bool any_brk = __ballot(brk) != 0ull;
+              // f = g = 0 (the underflow exit) runs as the rotation c = s = 0,
+              // which leaves d(i+1) - p; the exit's other effects follow below
+              T rn, cn, sn;
+              givens_nz(f, r, brk ? std::numeric_limits<T>::min() : ss, rn, cn, sn);
+              const T dix = d[i + 1] - delta;
+              const T q = (d[i] - dix) * sn + T(2) * cn * bb;
+              const T dx1 = fma_(sn, q, dix);
+              *lp++ = T2{cn, sn};
+              e[i + 1] = (i == hi) ? T(0) : rn;
+              d[i + 1] = dx1;
+              delta = dx1 - dix;
+              r = cn * q - bb;
+              c = cn;
+              s = sn;
+              d[i] = (i == lo) ? d[i] - delta : d[i];
+              e[i] = (i == lo) ? r : e[i];
+              if (__builtin_expect(any_brk, 0)) {
+                // the exit: no rotation logged, e(i+1) = 0, e(l) untouched,
+                // and the lane's sweep ends here
+                if (brk) {
+                  --lp;
+                  e[i + 1] = T(0);
+                  e[i] = ei;
+                  ilo = i + 1;
+                  lo = NB + 8;
+                }
+              }
+            }
           }
         }
       }
     }
+    nrot = (int)(lp - lg);
     if (sweep) {
       sw[nsw++] = sw_code(l, m, ilo);
       ++itr;
