@@ -443,8 +443,9 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
       if (8 * b + 8 > lmin) {
 #pragma unroll
         for (int i = 8 * b; i < 8 * b + 8 && i < NB - 1; ++i) {
-          const bool sm = !(abs_(e[i]) > (abs_(d[i]) + abs_(d[i + 1])) * tol);  // NaN counts as negligible
-          small.w[i >> 6] |= (sm && i < n - 1) ? (1ull << (i & 63)) : 0ull;
+          // NaN counts as negligible; the bits from n-1 on are set in top already
+          const bool sm = !(abs_(e[i]) > (abs_(d[i]) + abs_(d[i + 1])) * tol);
+          small.w[i >> 6] |= sm ? (1ull << (i & 63)) : 0ull;
         }
       }
     }
