@@ -180,6 +180,25 @@ __device__ __forceinline__ bool givens(double f, double g, double &r, double &c,
   return true;
 }
 
+// givens() for a known ss = f^2 + g^2 > 0 (the QL steps' form): the same
+// values, computed without the zero test.
+__device__ __forceinline__ void givens_nz(float f, float g, float ss, float &r, float &c, float &s)
+{
+  const float ri = rsqrt_nr(ss);
+  const float cn = g * ri, sn = f * ri;
+  const float h = -0.5f * __builtin_fmaf(cn, cn, __builtin_fmaf(sn, sn, -1.0f));
+  r = ss * ri;
+  c = __builtin_fmaf(cn, h, cn);
+  s = __builtin_fmaf(sn, h, sn);
+}
+__device__ __forceinline__ void givens_nz(double f, double g, double ss, double &r, double &c, double &s)
+{
+  const double ri = rsqrt_nr(ss);
+  r = ss * ri;
+  c = g * ri;
+  s = f * ri;
+}
+
 // ---- one value per lane of a group: element i lives in lane i ------------
 // Group-uniform index i.  For a full wave the element is read with
 // v_readlane (the index is wave-uniform, so it sits in an SGPR); for smaller

@@ -250,13 +250,19 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
       T r = (di1 - dl) + el / (f0 + sign_(g0, f0));
       T s = T(1), c = T(1), delta = T(0);
       T ei = eS[m - 1][lane], di = dS[m - 1][lane];
-      int i;
-      for (i = m - 1; i >= l; --i) {
+      T2 *lp = lg + nrot;
+      int j = l;                                   // where the sweep ends
+      for (int i = m - 1; i >= l; --i) {
         const int ip = (i > 0) ? i - 1 : 0;
         const T ein = eS[ip][lane], din = dS[ip][lane];
         const T f = s * ei, gg = r;
         const T bb = c * ei;
-        if (!givens(f, gg, r, c, s)) break;
+        const T ss = fma_(f, f, gg * gg);
+        const bool brk = ss == T(0);
+        const bool any_brk = __ballot(brk) != 0ull;
+        // f = g = 0 (the underflow exit) runs as the rotation c = s = 0, which
+        // stores d(i+1) - p; the exit's other effects follow below
+        givens_nz(f, gg, brk ? std::numeric_limits<T>::min() : ss, r, c, s);
         const T dix = di1 - delta;
         di1 = di;
         const T q = (di1 - dix) * s + T(2) * c * bb;
@@ -264,16 +270,27 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
         delta = dx1 - dix;
         eS[i + 1][lane] = r;
         dS[i + 1][lane] = dx1;
-        lg[nrot++] = T2{c, s};
+        *lp++ = T2{c, s};
         r = c * q - bb;
         ei = ein;
         di = din;
+        if (__builtin_expect(any_brk, 0)) {
+          // the exit: no rotation logged, e(i+1) = 0 (stored below), p
+          // already taken off d(i+1), and this lane's sweep ends here
+          if (brk) {
+            --lp;
+            r = T(0);
+            delta = T(0);
+            j = i + 1;
+            i = l;
+          }
+        }
       }
-      const int j = (i + 1 > l) ? i + 1 : l;
+      nrot = (int)(lp - lg);
       dS[j][lane] -= delta;
       eS[j][lane] = r;
       eS[m][lane] = T(0);
-      sw[nsw++] = sw_code(l, m, (i + 1 > l) ? i + 1 : l);
+      sw[nsw++] = sw_code(l, m, j);
     }
   }
 
@@ -309,26 +326,6 @@ __global__ __launch_bounds__(64) void eig_ql_lane_kernel(
     wout[i] = dS[i][lane];
     sout[(int)eS[i][lane]] = i < mcols ? i : -1;
   }
-}
-
-// Givens pair of the register QL step for ss = f^2 + g^2 > 0: r = sqrt(ss),
-// c = g/r, s = f/r from one refined reciprocal square root; in FP32 (c, s)
-// renormalised once with the fma-exact residual c^2 + s^2 - 1.
-__device__ __forceinline__ void givens_nz(float f, float g, float ss, float &r, float &c, float &s)
-{
-  const float ri = rsqrt_nr(ss);
-  const float cn = g * ri, sn = f * ri;
-  const float h = -0.5f * __builtin_fmaf(cn, cn, __builtin_fmaf(sn, sn, -1.0f));
-  r = ss * ri;
-  c = __builtin_fmaf(cn, h, cn);
-  s = __builtin_fmaf(sn, h, sn);
-}
-__device__ __forceinline__ void givens_nz(double f, double g, double ss, double &r, double &c, double &s)
-{
-  const double ri = rsqrt_nr(ss);
-  r = ss * ri;
-  c = g * ri;
-  s = f * ri;
 }
 
 // Bit mask over the off-diagonal indices 0 .. NB-2 of the register QL kernel:
