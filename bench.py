@@ -858,6 +858,7 @@ def main():
     ap.add_argument("--ws-total-steps", type=int, default=200, help="warm-start line: stream length")
     ap.add_argument("--ws-eval-steps", type=int, nargs="+", default=[50, 100, 150, 199],
                     help="warm-start line: evaluation steps")
+    ap.add_argument("--no-chunks", action="store_true", help="skip the chunk trade-off line")
     ap.add_argument("--no-mixed", action="store_true",
                     help="n256 line without the mixed-precision and chunk sub-lines (for profiling)")
     ap.add_argument("--mixed-only", action="store_true",
@@ -966,7 +967,7 @@ def main():
         "accuracy": acc,
     }
     out["accuracy_all"] = accuracy_all(a0, a, w)
-    if n <= 64 and L >= 2048:
+    if n <= 64 and L >= 2048 and not args.no_chunks:
         out["chunk_tradeoff"] = chunk_tradeoff(a0, (0, 1, 2, 3), steps=args.steps, warmup=args.warmup)
     if not args.no_tuned:
         out["tuned"] = tuned_line(a0, ms, args.steps, args.warmup)
