@@ -551,23 +551,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TriWaves<T, 
   }
 }
 
-template <typename T, int NCH, int CW>
+template <typename T, int NCH, int CW, int RPL>
 struct alignas(16) ApplyLargeLds {
-  alignas(16) T red[2][NCH][64];                 // per-wave partial dot products (phase A)
-  alignas(16) T topv[NCH][64];                   // wave q's top column after its last sweep
-  alignas(16) T carry[NCH][64];                  // pending chain value handed to wave q
+  alignas(16) T red[2][NCH][RPL][64];            // per-wave partial dot products (phase A)
+  alignas(16) T topv[NCH][RPL][64];              // wave q's top column after its last sweep
+  alignas(16) T carry[NCH][RPL][64];             // pending chain value handed to wave q
   alignas(16) T us[NCH][CW];                     // reflector entries of wave q's columns
   alignas(16) typename vec2<T>::type rs[NCH][CW];  // rotations of wave q's current sweep
   alignas(16) int pos[NCH][CW];
   alignas(16) T beta[GL_NMAX];
 };
 
-// CW columns per wave (32, or 16 for FP64 so that the 16 doubles of a row
-// chunk and the chain fit next to each other in registers)
-// FP64 is held to 128 VGPRs (four waves per SIMD: two 8-wave workgroups per
-// CU); FP32 keeps the registers its branch-free chain wants
-// occupancy floor (waves per SIMD) of the apply kernel: held to 128 VGPRs, two
-// 8-wave workgroups per CU
+// occupancy floor of the apply kernel (waves per SIMD): held to 128 VGPRs
 template <typename T, int NCH> struct ApWaves { static constexpr int value = 4; };
 
 // LIST: the uneven-batch form (u), one workgroup per (slot, row block of
@@ -575,7 +570,11 @@ template <typename T, int NCH> struct ApWaves { static constexpr int value = 4; 
 // SKIPQ: start from the identity instead of Q (phase A skipped): the result is
 // the tridiagonal's eigenvector matrix Z, for a separate back-transformation
 // (eig_wy.hpp).
-template <typename T, int NCH, int CW, bool LIST = false, bool SKIPQ = false>
+// RPL rows per lane: a workgroup takes a block of 64 RPL rows, lane l of every
+// wave the rows 64 rr + l (rr < RPL) of the block.  The rows share each
+// rotation's LDS read and run as RPL independent chains, so a sweep's chain
+// through the wave's CW columns is RPL times as wide and no longer.
+template <typename T, int NCH, int CW, bool LIST = false, bool SKIPQ = false, int RPL = 1>
 __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWaves<T, NCH>::value))) void eig_apply_large_kernel(
     int L, int nm, int n, T *__restrict__ a, const T *__restrict__ W, const T *__restrict__ bws,
     const typename vec2<T>::type *__restrict__ rotlog, int rot_cap, const int *__restrict__ swlog, int sw_cap,
@@ -583,8 +582,9 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
     UvSplitArgs u = UvSplitArgs{})
 {
   typedef typename vec2<T>::type T2;
-  __shared__ ApplyLargeLds<T, NCH, CW> S;
-  const int nrb = ((LIST ? u.nmx : n) + 63) >> 6;
+  __shared__ ApplyLargeLds<T, NCH, CW, RPL> S;
+  constexpr int RB = 64 * RPL;                       // rows per workgroup
+  const int nrb = ((LIST ? u.nmx : n) + RB - 1) / RB;
   const long long mat = blockIdx.x / nrb;            // the slot (LIST) or the matrix
   const int rb = blockIdx.x - (int)(mat * nrb);
   size_t wstride = (size_t)n * n, bstride = n, sstride = n;
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
     if (!uv_split_entry(u, mat, mi)) return;
     n = __builtin_amdgcn_readfirstlane(u.nv[mi]);
     nm = n;
-    if (rb * 64 >= n) return;
+    if (rb * RB >= n) return;
     wstride = (size_t)u.nmx * u.nmx;
     bstride = sstride = u.nmx;
     A = a + u.aoff[mi];
@@ -604,14 +604,18 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
   }
   if (status[mat] != 0) return;                      // overflow: solved by the fallback
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int r = rb * 64 + lane;                      // row of Q handled by this lane
   const int c0 = q * CW;                             // first column of this wave
   const T *__restrict__ Wm = W + (size_t)mat * wstride;
   const T *__restrict__ beta = bws + (size_t)mat * bstride;
-
-  T x[CW];
+  int r[RPL];                                        // rows of Q handled by this lane
 #pragma unroll
-  for (int j = 0; j < CW; ++j) x[j] = (r == c0 + j) ? T(1) : T(0);
+  for (int rr = 0; rr < RPL; ++rr) r[rr] = rb * RB + rr * 64 + lane;
+
+  T x[RPL][CW];
+#pragma unroll
+  for (int rr = 0; rr < RPL; ++rr)
+#pragma unroll
+    for (int j = 0; j < CW; ++j) x[rr][j] = (r[rr] == c0 + j) ? T(1) : T(0);
 
   // ---- A. rows of Q = H_{n-1} ... H_1: x <- x H_c, c = n-1 down to 1 --------
   // (the right-multiplication of wave_front, split over the column chunks)
@@ -624,22 +628,40 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
     const T ucur = (c0 + lane < c) ? unext : T(0);
     unext = unext2;
     if (c > 2 && lane < CW) unext2 = (c0 + lane < c - 2) ? Wm[c0 + lane + (size_t)(c - 2) * n] : T(0);
-    T part = T(0);
+    T part[RPL];
+#pragma unroll
+    for (int rr = 0; rr < RPL; ++rr) part[rr] = T(0);
     if (c0 < c) {
       if (lane < CW) S.us[q][lane] = ucur;
       wsync();
 #pragma unroll
-      for (int j = 0; j < CW; ++j) part = fma_(x[j], S.us[q][j], part);
-    }
-    S.red[c & 1][q][lane] = part;
-    __syncthreads();
-    T tsum = T(0);
+      for (int j = 0; j < CW; ++j) {
+        const T uj = S.us[q][j];
 #pragma unroll
-    for (int w2 = 0; w2 < NCH; ++w2) tsum += S.red[c & 1][w2][lane];
-    tsum *= S.beta[c];
+        for (int rr = 0; rr < RPL; ++rr) part[rr] = fma_(x[rr][j], uj, part[rr]);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < RPL; ++rr) S.red[c & 1][q][rr][lane] = part[rr];
+    __syncthreads();
+    T tsum[RPL];
+#pragma unroll
+    for (int rr = 0; rr < RPL; ++rr) tsum[rr] = T(0);
+    // a few waves' partials in flight at a time (all NCH RPL of them at once
+    // would take the registers the rows need)
+#pragma unroll 4
+    for (int w2 = 0; w2 < NCH; ++w2)
+#pragma unroll
+      for (int rr = 0; rr < RPL; ++rr) tsum[rr] += S.red[c & 1][w2][rr][lane];
+#pragma unroll
+    for (int rr = 0; rr < RPL; ++rr) tsum[rr] *= S.beta[c];
     if (c0 < c) {
 #pragma unroll
-      for (int j = 0; j < CW; ++j) x[j] = fma_(tsum, S.us[q][j], x[j]);
+      for (int j = 0; j < CW; ++j) {
+        const T uj = S.us[q][j];
+#pragma unroll
+        for (int rr = 0; rr < RPL; ++rr) x[rr][j] = fma_(tsum[rr], uj, x[rr][j]);
+      }
     }
     wsync();
   }
@@ -664,7 +686,8 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
   int code = (nsw > 0) ? sw[0] : 0;
   int code1 = (nsw > 1) ? sw[1] : 0;
   T2 pre = (nsw > 0) ? fetch(code, 0) : T2{T(1), T(0)};
-  S.topv[q][lane] = x[CW - 1];
+#pragma unroll
+  for (int rr = 0; rr < RPL; ++rr) S.topv[q][rr][lane] = x[rr][CW - 1];
   __syncthreads();
   const int nsteps = (nsw > 0) ? 2 * (nsw - 1) + NCH : 0;
   int s = 0;                                         // next sweep of this wave
@@ -682,24 +705,26 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
       wsync();
       // 4-pair blocks from the one holding the window's top pair down to
       // the one holding its bottom pair; the rotations of a block are read
-      // together from LDS as the block starts (prefetching the next block's
-      // into a second register set cost more in moves and spills than the
-      // LDS latency it hid), and the identity rotations outside the window
-      // make every pair of a block safe to apply.
+      // together from LDS as the block starts, and the identity rotations
+      // outside the window make every pair of a block safe to apply.
       if (enters || top >= bot) {
         constexpr int PB = 4;
         const int ptop = enters ? CW - 1 : top - (c0 - 1);   // highest pair index p to apply
         const int pbot = bot - (c0 - 1);                      // lowest pair index p in the window
         const int btop = ptop / PB, bbot = pbot / PB;
-        T h1 = T(0);
-        if (enters) {
-          h1 = S.carry[q][lane];
-        } else {
+        T h1[RPL], xlow[RPL];
 #pragma unroll
-          for (int bb = 0; bb < CW / PB; ++bb)                  // x[btop PB + PB - 1]
-            if (bb == btop) h1 = x[bb * PB + PB - 1];
+        for (int rr = 0; rr < RPL; ++rr) {
+          h1[rr] = T(0);
+          if (enters) {
+            h1[rr] = S.carry[q][rr][lane];
+          } else {
+#pragma unroll
+            for (int bb = 0; bb < CW / PB; ++bb)                // x[btop PB + PB - 1]
+              if (bb == btop) h1[rr] = x[rr][bb * PB + PB - 1];
+          }
+          xlow[rr] = (q > 0) ? S.topv[q - 1][rr][lane] : T(0);
         }
-        const T xlow = (q > 0) ? S.topv[q - 1][lane] : T(0);
         T2 g[PB];
 #pragma unroll
         for (int b = CW / PB - 1; b >= 0; --b) {
@@ -709,13 +734,16 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
 #pragma unroll
             for (int jj = PB - 1; jj >= 0; --jj) {
               const int p = b * PB + jj;
-              if (p >= 1) {
-                const T h0 = x[p >= 1 ? p - 1 : 0];
-                x[p] = g[jj].y * h0 + g[jj].x * h1;
-                h1 = g[jj].x * h0 - g[jj].y * h1;
-              } else if (q > 0) {
-                x[0] = g[jj].y * xlow + g[jj].x * h1;
-                h1 = g[jj].x * xlow - g[jj].y * h1;
+#pragma unroll
+              for (int rr = 0; rr < RPL; ++rr) {
+                if (p >= 1) {
+                  const T h0 = x[rr][p >= 1 ? p - 1 : 0];
+                  x[rr][p] = g[jj].y * h0 + g[jj].x * h1[rr];
+                  h1[rr] = g[jj].x * h0 - g[jj].y * h1[rr];
+                } else if (q > 0) {
+                  x[rr][0] = g[jj].y * xlow[rr] + g[jj].x * h1[rr];
+                  h1[rr] = g[jj].x * xlow[rr] - g[jj].y * h1[rr];
+                }
               }
             }
           }
@@ -725,14 +753,19 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
         // pair applied, its pending value is the final column ilo
         const int plow = (bbot * PB > (q > 0 ? 0 : 1)) ? bbot * PB : (q > 0 ? 0 : 1);
         if (q > 0 && pbot == 0) {
-          S.carry[q - 1][lane] = h1;
+#pragma unroll
+          for (int rr = 0; rr < RPL; ++rr) S.carry[q - 1][rr][lane] = h1[rr];
         } else if (plow == pbot && bot == ilo) {
 #pragma unroll
           for (int j = 0; j < CW; ++j)
-            if (j == pbot - 1) x[j] = h1;
+            if (j == pbot - 1) {
+#pragma unroll
+              for (int rr = 0; rr < RPL; ++rr) x[rr][j] = h1[rr];
+            }
         }
       }
-      S.topv[q][lane] = x[CW - 1];
+#pragma unroll
+      for (int rr = 0; rr < RPL; ++rr) S.topv[q][rr][lane] = x[rr][CW - 1];
       base += cnt;
       ++s;
       code = code1;
@@ -744,10 +777,13 @@ __global__ __launch_bounds__(64 * NCH) __attribute__((amdgpu_waves_per_eu(ApWave
   // ---- C. permuted store: column j goes to slot[j] --------------------------
   if (lane < CW) S.pos[q][lane] = (c0 + lane < n) ? slot[(size_t)mat * sstride + c0 + lane] : 0;
   wsync();
-  if (r < n) {
 #pragma unroll
-    for (int j = 0; j < CW; ++j)
-      if (c0 + j < n && S.pos[q][j] >= 0) A[r + (size_t)S.pos[q][j] * nm] = x[j];
+  for (int rr = 0; rr < RPL; ++rr) {
+    if (r[rr] < n) {
+#pragma unroll
+      for (int j = 0; j < CW; ++j)
+        if (c0 + j < n && S.pos[q][j] >= 0) A[r[rr] + (size_t)S.pos[q][j] * nm] = x[rr][j];
+    }
   }
 }
 

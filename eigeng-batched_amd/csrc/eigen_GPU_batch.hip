@@ -347,13 +347,15 @@ hipError_t launch_global(int L, int nm, int n, T *a, T *w, int *info, int solver
   return hipGetLastError();
 }
 
-template <typename T, int NCH, int CW, bool SKIPQ = false>
-void launch_apply_large(unsigned grid, int L, int nm, int n, T *a, const T *Wm, const T *bw,
+// one workgroup per block of 64 RPL rows of each of the L matrices
+template <typename T, int NCH, int CW, bool SKIPQ = false, int RPL = 1>
+void launch_apply_large(int L, int nm, int n, T *a, const T *Wm, const T *bw,
                         const typename vec2<T>::type *rot, int rcap, const int *sw, int scap, const int *nsw,
                         const int *st, const int *slot, hipStream_t s)
 {
-  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH, CW, false, SKIPQ>), dim3(grid), dim3(64 * NCH), 0, s, L, nm, n,
-                     a, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, UvSplitArgs{});
+  const unsigned grid = (unsigned)L * (unsigned)((n + 64 * RPL - 1) / (64 * RPL));
+  hipLaunchKernelGGL((eig_apply_large_kernel<T, NCH, CW, false, SKIPQ, RPL>), dim3(grid), dim3(64 * NCH), 0, s, L,
+                     nm, n, a, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, UvSplitArgs{});
 }
 
 template <typename T>
@@ -506,14 +508,13 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
                          cur, n, de, lc, rot, rcap, sw, scap, nsw, st, slot, wc, ic, sv);
     }
     if (!(ph & FLAG_PH_APPLY)) continue;
-    const unsigned ag = (unsigned)cur * (unsigned)((n + 63) / 64);
     constexpr int CWS = 32;  // chunk width
     if constexpr (sizeof(T) == 4) {
       if (wyb) {
         if (n <= 128)
-          launch_apply_large<T, 128 / CWS, CWS, true>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+          launch_apply_large<T, 128 / CWS, CWS, true>(cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
         else
-          launch_apply_large<T, 256 / CWS, CWS, true>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+          launch_apply_large<T, 256 / CWS, CWS, true>(cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
         hipError_t e2 = wy_backtransform(cur, nm, n, Wm, bw, st, ac, wyb, s);
         if (e2 != hipSuccess) return e2;
         hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(tg), dim3(GL_THREADS), 0, s, cur, nm, n, ac, wc,
@@ -521,14 +522,17 @@ hipError_t launch_split_large(int L, int nm, int n, T *a, T *w, int *info, int s
         continue;
       }
     }
+    // FP32 up to n = 256: two rows per lane (two chains per wave sharing each
+    // rotation's LDS read)
+    constexpr int RPL = sizeof(T) == 4 ? 2 : 1;
     if (n <= 128)
-      launch_apply_large<T, 128 / CWS, CWS>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 128 / CWS, CWS, false, RPL>(cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else if (n <= 256)
-      launch_apply_large<T, 256 / CWS, CWS>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 256 / CWS, CWS, false, RPL>(cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else if (n <= 384)
-      launch_apply_large<T, 12, 32>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 12, 32>(cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     else
-      launch_apply_large<T, 16, 32>(ag, cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
+      launch_apply_large<T, 16, 32>(cur, nm, n, ac, Wm, bw, rot, rcap, sw, scap, nsw, st, slot, s);
     // overflow fallback: the workgroup solver on the flagged matrices, with
     // the (now free) tridiagonalisation slots as its accumulators
     hipLaunchKernelGGL((eig_global_kernel<T, GL_THREADS>), dim3(tg), dim3(GL_THREADS), 0, s, cur, nm, n, ac, wc,
