@@ -9,11 +9,12 @@
 //                          solve any matrix whose rotation log overflows.
 //   eig_ql_lane_kernel     (eig_split.hpp, NLANE matrices per wave): the QL
 //                          recurrence with rotation and sweep logs, sort, w.
-//   eig_apply_large_kernel (workgroup per 64-row block of a matrix): rows of
-//                          Q = H_{n-1} ... H_1 formed in registers, then every
-//                          logged sweep replayed on them, then the permuted store.
+//   eig_apply_large_kernel (workgroup per block of 64 RPL rows of a matrix):
+//                          rows of Q = H_{n-1} ... H_1 formed in registers, then
+//                          every logged sweep replayed on them, then the
+//                          permuted store.
 //
-// Apply kernel layout: lane = row of the block, wave q = CW-column chunk
+// Apply kernel layout: lane = row of the block (RPL rows per lane), wave q = CW-column chunk
 // [CW q, CW q + CW-1] held in registers (the description below uses CW = 32).  Rotations act on columns, so rows are
 // independent, but one sweep chains through the columns from the top of its
 // window down: wave q handles the pairs i in [32q-1, 32q+30] (pair i finalises
