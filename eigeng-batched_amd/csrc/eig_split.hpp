@@ -608,7 +608,10 @@ __global__ __launch_bounds__(64) void eig_ql_reg_kernel(
 
 template <typename T, int NB>
 struct alignas(16) ApplyLds {
-  alignas(16) typename vec2<T>::type cs[NB];
+  // sweeps in flight: two for FP32 NB = 128 (apply_two_sweeps; it measured
+  // slower in FP64 and no faster at NB = 64), one otherwise
+  static constexpr int NS = (sizeof(T) == 4 && NB == 128) ? 2 : 1;
+  alignas(16) typename vec2<T>::type cs[NS][NB];   // rotations of the sweeps in flight
   alignas(16) int pos[NB];
 };
 
@@ -617,8 +620,16 @@ struct alignas(16) ApplyLds {
 // row of 128 columns; the blocks share nothing, every wave stages the
 // rotations of a sweep (up to NB - 1) itself, NR per lane.
 // LIST (NB = 64 and 128): the uneven-batch form (u), slot = blockIdx / NR.
+// FP32 NB = 128 replays the sweeps in pairs, the second sweep one 8-block
+// behind the first (apply_two_sweeps), so that two rotation chains overlap.
+// occupancy floor of the apply kernel (waves per SIMD) with two sweeps in
+// flight: the row in registers plus two sweeps' rotation blocks
+template <typename T, int NB> struct ApplyWaves {
+  static constexpr int value = (ApplyLds<T, NB>::NS == 2) ? 3 : 1;
+};
+
 template <typename T, int NB, bool LIST = false>
-__global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *__restrict__ a,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ApplyWaves<T, NB>::value))) void eig_apply_kernel(int L, int nm, int n, T *__restrict__ a,
                                                        const T *__restrict__ qws,
                                                        const typename vec2<T>::type *__restrict__ rotlog,
                                                        int rot_cap, const int *__restrict__ swlog, int sw_cap,
@@ -659,44 +670,104 @@ __global__ __launch_bounds__(64) void eig_apply_kernel(int L, int nm, int n, T *
   T R[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) R[j] = (j < n && k < n) ? Q[k + (size_t)j * n] : T(0);
-#pragma unroll
-  for (int t = 0; t < NR; ++t) S.cs[kl + NL * t] = T2{T(1), T(0)};
-  const int nsw = nsw_in[mat];
-  // software pipeline: the next sweep's header and this lane's log entries
-  // (positions kl + NL t of the sweep) are loaded while the current chain runs
-  int code = (nsw > 0) ? sw[0] : 0;
-  int base = 0;
-  T2 q[NR];
-  {
-    const int m = sw_m(code), ilo = sw_ilo(code);
-#pragma unroll
-    for (int t = 0; t < NR; ++t) q[t] = (nsw > 0 && kl + NL * t < m - ilo) ? lg[kl + NL * t] : T2{T(1), T(0)};
-  }
-  for (int s = 0; s < nsw; ++s) {
-    const int m = sw_m(code), ilo = sw_ilo(code);
-    const int cnt = m - ilo;
-#pragma unroll
-    for (int t = 0; t < NR; ++t)
-      if (kl + NL * t < cnt) S.cs[m - 1 - (kl + NL * t)] = q[t];
-    base += cnt;
-    // prefetch sweep s+1
-    const int code2 = (s + 1 < nsw) ? sw[s + 1] : 0;
-    {
-      const int m2 = sw_m(code2), ilo2 = sw_ilo(code2);
-#pragma unroll
-      for (int t = 0; t < NR; ++t)
-        if (s + 1 < nsw && kl + NL * t < m2 - ilo2) q[t] = lg[base + kl + NL * t];
-    }
-    wsync();
-    apply_sweep_rotations<T, NB>(R, S.cs, ilo, m - 1);
-    wsync();
+  if constexpr (ApplyLds<T, NB>::NS == 2) {
 #pragma unroll
     for (int t = 0; t < NR; ++t) {
-      const int i = kl + NL * t;
-      if (i >= ilo && i <= m - 1) S.cs[i] = T2{T(1), T(0)};
+      S.cs[0][kl + NL * t] = T2{T(1), T(0)};
+      S.cs[1][kl + NL * t] = T2{T(1), T(0)};
     }
-    code = code2;
-    wsync();
+    const int nsw = nsw_in[mat];
+    // sweeps in pairs (the second one block behind the first, apply_two_sweeps);
+    // software pipeline: the next pair's headers and this lane's log entries
+    // (positions kl + NL t of each sweep) are loaded while the current pair runs
+    int code0 = (nsw > 0) ? sw[0] : 0, code1 = (nsw > 1) ? sw[1] : 0;
+    int base = 0;
+    T2 q0[NR], q1[NR];
+    {
+      const int cnt0 = sw_m(code0) - sw_ilo(code0), cnt1 = sw_m(code1) - sw_ilo(code1);
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        q0[t] = (nsw > 0 && kl + NL * t < cnt0) ? lg[kl + NL * t] : T2{T(1), T(0)};
+        q1[t] = (nsw > 1 && kl + NL * t < cnt1) ? lg[cnt0 + kl + NL * t] : T2{T(1), T(0)};
+      }
+    }
+    for (int s = 0; s < nsw; s += 2) {
+      const bool two = s + 1 < nsw;
+      const int m0 = sw_m(code0), ilo0 = sw_ilo(code0), cnt0 = m0 - ilo0;
+      const int m1 = sw_m(code1), ilo1 = sw_ilo(code1), cnt1 = two ? m1 - ilo1 : 0;
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        if (kl + NL * t < cnt0) S.cs[0][m0 - 1 - (kl + NL * t)] = q0[t];
+        if (kl + NL * t < cnt1) S.cs[1][m1 - 1 - (kl + NL * t)] = q1[t];
+      }
+      base += cnt0 + cnt1;
+      // prefetch sweeps s+2, s+3
+      const int c2 = (s + 2 < nsw) ? sw[s + 2] : 0, c3 = (s + 3 < nsw) ? sw[s + 3] : 0;
+      {
+        const int n2 = (s + 2 < nsw) ? sw_m(c2) - sw_ilo(c2) : 0;
+        const int n3 = (s + 3 < nsw) ? sw_m(c3) - sw_ilo(c3) : 0;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) {
+          if (kl + NL * t < n2) q0[t] = lg[base + kl + NL * t];
+          if (kl + NL * t < n3) q1[t] = lg[base + n2 + kl + NL * t];
+        }
+      }
+      wsync();
+      // an odd last sweep goes with an empty second window (its blocks are
+      // no-ops: S.cs[1] holds identities only); one code path keeps the row in
+      // the same registers
+      apply_two_sweeps<T, NB>(R, S.cs[0], ilo0, m0 - 1, S.cs[1], two ? ilo1 : NB + 8, two ? m1 - 1 : -1);
+      wsync();
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        const int i = kl + NL * t;
+        if (i >= ilo0 && i <= m0 - 1) S.cs[0][i] = T2{T(1), T(0)};
+        if (two && i >= ilo1 && i <= m1 - 1) S.cs[1][i] = T2{T(1), T(0)};
+      }
+      code0 = c2;
+      code1 = c3;
+      wsync();
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < NR; ++t) S.cs[0][kl + NL * t] = T2{T(1), T(0)};
+    const int nsw = nsw_in[mat];
+    // software pipeline: the next sweep's header and this lane's log entries
+    // (positions kl + NL t of the sweep) are loaded while the current chain runs
+    int code = (nsw > 0) ? sw[0] : 0;
+    int base = 0;
+    T2 q[NR];
+    {
+      const int m = sw_m(code), ilo = sw_ilo(code);
+#pragma unroll
+      for (int t = 0; t < NR; ++t) q[t] = (nsw > 0 && kl + NL * t < m - ilo) ? lg[kl + NL * t] : T2{T(1), T(0)};
+    }
+    for (int s = 0; s < nsw; ++s) {
+      const int m = sw_m(code), ilo = sw_ilo(code);
+      const int cnt = m - ilo;
+#pragma unroll
+      for (int t = 0; t < NR; ++t)
+        if (kl + NL * t < cnt) S.cs[0][m - 1 - (kl + NL * t)] = q[t];
+      base += cnt;
+      // prefetch sweep s+1
+      const int code2 = (s + 1 < nsw) ? sw[s + 1] : 0;
+      {
+        const int m2 = sw_m(code2), ilo2 = sw_ilo(code2);
+#pragma unroll
+        for (int t = 0; t < NR; ++t)
+          if (s + 1 < nsw && kl + NL * t < m2 - ilo2) q[t] = lg[base + kl + NL * t];
+      }
+      wsync();
+      apply_sweep_rotations<T, NB>(R, S.cs[0], ilo, m - 1);
+      wsync();
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        const int i = kl + NL * t;
+        if (i >= ilo && i <= m - 1) S.cs[0][i] = T2{T(1), T(0)};
+      }
+      code = code2;
+      wsync();
+    }
   }
 #pragma unroll
   for (int t = 0; t < NR; ++t)

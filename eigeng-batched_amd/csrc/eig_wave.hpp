@@ -130,6 +130,49 @@ __device__ __forceinline__ void apply_sweep_rotations(T (&Zr)[NB], const typenam
   }
 }
 
+// Two consecutive sweeps A then B, B one 8-block behind A: in round b, A's
+// pairs of block b and then B's pairs of block b+1.  B's pair j needs columns
+// j and j+1 as A leaves them, which A's pairs j-1 and j (blocks at or above
+// b) have finished, so every element sees A's rotations before B's exactly
+// as in two apply_sweep_rotations calls, and the two chains of a round sit in
+// one basic block so that their latencies overlap.  A round runs both blocks
+// whenever either is inside its sweep's window; a block outside its own
+// window is a no-op: its rotations are the identity, and its chain restarts
+// at the block's top and commits at its bottom.
+template <typename T, int NB>
+__device__ __forceinline__ void apply_two_sweeps(T (&Zr)[NB], const typename vec2<T>::type *csA, int loA, int hiA,
+                                                 const typename vec2<T>::type *csB, int loB, int hiB)
+{
+  typedef typename vec2<T>::type T2;
+  constexpr int NBLK = NB / 8;
+  const int bloA = loA >> 3, bhiA = hiA >> 3, bloB = loB >> 3, bhiB = hiB >> 3;
+  // rounds b with A's block b or B's block b+1 in its window
+  const int rlo = (bloA < bloB - 1) ? bloA : bloB - 1, rhi = (bhiA > bhiB - 1) ? bhiA : bhiB - 1;
+  T hA = T(0), hB = T(0);
+  auto blk = [&](const T2 *cs, T &h1, int b, int blo, int bhi) {
+    const int itop = (8 * b + 7 < NB - 1) ? 8 * b + 7 : NB - 2;
+    h1 = (b >= bhi) ? Zr[itop + 1] : h1;
+#pragma unroll
+    for (int i = itop; i >= 8 * b; --i) {
+      const T2 q = cs[i];
+      const T h0 = Zr[i];
+      Zr[i + 1] = q.y * h0 + q.x * h1;
+      h1 = q.x * h0 - q.y * h1;
+    }
+    Zr[8 * b] = (b <= blo) ? h1 : Zr[8 * b];
+  };
+#pragma unroll
+  for (int b = NBLK - 1; b >= -1; --b) {
+    if (b >= rlo && b <= rhi) {
+      // keeps each round's rotation loads in their round (hoisted ahead, the
+      // loads of later rounds would hold far more registers than the row)
+      asm volatile("" ::: "memory");
+      if (b >= 0) blk(csA, hA, b >= 0 ? b : 0, bloA, bhiA);
+      if (b + 1 <= NBLK - 1) blk(csB, hB, b + 1 <= NBLK - 1 ? b + 1 : NBLK - 1, bloB, bhiB);
+    }
+  }
+}
+
 // Front half shared by the fused and the split solvers: load row k of matrix A
 // (upper triangle only), Householder tridiagonalisation, Q = H_{n-1}...H_1 in R.
 // On return R holds row k of Q and dv / ev the lane-distributed diagonal and
