@@ -28,7 +28,8 @@
 // the 32 x 32 quarter (q & 1, q >> 1), i.e. 2 x 2 MFMA tiles of 16 x 16.  The
 // contraction runs in chunks of 16: the next chunk's global loads are issued
 // before the current chunk's MFMAs, then stored to LDS (k-major, so that the
-// lanes of an MFMA operand read consecutive words).  Out-of-range rows, columns
+// lanes of an MFMA operand read consecutive words), into one of two LDS buffers
+// while the other one is multiplied.  Out-of-range rows, columns
 // and k are zero-filled, so any M, N, K >= 1 is valid.
 //
 // sym > 0: C_b is a row of `sym`-wide square blocks of tiles whose results are
@@ -245,15 +246,19 @@ __device__ __forceinline__ long long gemm_xcd_remap(long long bid, long long tot
 // chunk.  Non-transposed A (rows contiguous): thread t takes row t & 63 and k =
 // (t >> 6) + 4 i; transposed A (k contiguous): k = t & 15, row = (t >> 4) + 16 i.
 // B likewise with the roles of rows and k exchanged.
+// held to 128 registers (four waves per SIMD): with the second chunk buffer
+// the FP64 kind otherwise takes 136 and runs at three
 template <int KIND>
-__global__ __launch_bounds__(GM_THREADS) void eig_gemm_kernel(GemmArgs g)
+__global__ __launch_bounds__(GM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void eig_gemm_kernel(GemmArgs g)
 {
   typedef GemmT<KIND> TT;
   typedef typename TT::S S;
   typedef typename TT::L Ls;
   typedef typename TT::v4 v4;
-  __shared__ Ls As[GM_KC][GM_TILE + GM_PAD];
-  __shared__ Ls Bs[GM_KC][GM_TILE + GM_PAD];
+  // two chunk buffers: the next chunk is stored while the current one is
+  // multiplied, with one barrier per chunk
+  __shared__ Ls As[2][GM_KC][GM_TILE + GM_PAD];
+  __shared__ Ls Bs[2][GM_KC][GM_TILE + GM_PAD];
 
   const int tm = (g.M + GM_TILE - 1) / GM_TILE, tn = (g.N + GM_TILE - 1) / GM_TILE;
   long long tiles;
@@ -327,20 +332,32 @@ __global__ __launch_bounds__(GM_THREADS) void eig_gemm_kernel(GemmArgs g)
   S ra[4], rb[4];
   loadA(0, ra);
   loadB(0, rb);
-  for (int k0 = 0; k0 < g.K; k0 += GM_KC) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      As[ak[i]][ar[i]] = TT::cvt(ra[i]);
-      Bs[bk[i]][bc[i]] = TT::cvt(rb[i]);
-    }
-    __syncthreads();
-    if (k0 + GM_KC < g.K) {
+  for (int i = 0; i < 4; ++i) {
+    As[0][ak[i]][ar[i]] = TT::cvt(ra[i]);
+    Bs[0][bk[i]][bc[i]] = TT::cvt(rb[i]);
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < g.K; k0 += GM_KC) {
+    // chunk k0 + GM_KC: global loads before, LDS stores after this chunk's
+    // products; the buffer it goes to was last read before the previous barrier
+    const bool more = k0 + GM_KC < g.K;
+    if (more) {
       loadA(k0 + GM_KC, ra);
       loadB(k0 + GM_KC, rb);
     }
-    if constexpr (KIND == GK_F16X3) GemmMma<KIND>::chunk(As, Bs, wr, wc, m, gg, acc, acc2);
-    else GemmMma<KIND>::chunk(As, Bs, wr, wc, m, gg, acc);
+    if constexpr (KIND == GK_F16X3) GemmMma<KIND>::chunk(As[cur], Bs[cur], wr, wc, m, gg, acc, acc2);
+    else GemmMma<KIND>::chunk(As[cur], Bs[cur], wr, wc, m, gg, acc);
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        As[cur ^ 1][ak[i]][ar[i]] = TT::cvt(ra[i]);
+        Bs[cur ^ 1][bk[i]][bc[i]] = TT::cvt(rb[i]);
+      }
+    }
     __syncthreads();
+    cur ^= 1;
   }
 
   // epilogue: C = alpha acc + beta D
