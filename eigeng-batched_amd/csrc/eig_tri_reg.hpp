@@ -171,6 +171,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
     if (tid == 0) S.d[nv - 1] = S.y[nv & 1][nv - 1];
 
     for (int c = nv - 1; c >= 1; --c) {
+      // the lane-derived indices are formed again for every column: held from
+      // before the loop they were spilled, and their reloads from scratch sat
+      // on the column's critical path
+      int lane_c = lane;
+      asm volatile("" : "+v"(lane_c));
+      const int rr = lane_c & 31, hh = lane_c >> 5, tid_c = wv * 64 + lane_c;
       // 1. norm, per-tile row and column sums of A11 x (x in this wave's copy)
       wsync();
       const T sg = group_sum<64>(sgl);
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
       __syncthreads();                                      // tile sums complete
       // 2. p_i (thread i), u_i, the partial sums of u . p
       if (wv < nsum) {
-        const int i = tid;
+        const int i = tid_c;
         const int b = i >> 5, r = i & 31;
         T s = T(0);
 #pragma unroll
@@ -231,9 +237,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
         S.p[i] = pi;
         S.u[i] = ui;
         const T pr = group_sum<64>(pi * ui);
-        if (lane == 0) S.red[wv] = pr;
+        if (lane_c == 0) S.red[wv] = pr;
       }
-      if (tid == 0) {
+      if (tid_c == 0) {
         S.e[c - 1] = anorm;
         S.beta[c] = beta;
       }
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
       alpha *= beta * T(0.5);
       const T alpha2 = alpha + alpha;
       if (wv == c % NW)
-        for (int r = lane; r < c; r += 64) Wm[r + (size_t)c * n] = S.u[r];
+        for (int r = lane_c; r < c; r += 64) Wm[r + (size_t)c * n] = S.u[r];
       // 3. rank-2 update of the active tiles
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
@@ -262,8 +268,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
       // d_{c-1} likewise
       const T uc = S.u[c - 1];
       const T wc = fma_(alpha2, uc, S.p[c - 1]);
-      sgl = tr_form_x<T, NW>(c - 1, uc, wc, S.y[c & 1], xw, S, lane);
-      if (tid == 0) S.d[c - 1] = fma_(wc, uc, fma_(uc, S.p[c - 1], S.y[c & 1][c - 1]));
+      sgl = tr_form_x<T, NW>(c - 1, uc, wc, S.y[c & 1], xw, S, lane_c);
+      if (tid_c == 0) S.d[c - 1] = fma_(wc, uc, fma_(uc, S.p[c - 1], S.y[c & 1][c - 1]));
     }
     __syncthreads();
     for (int i = tid; i < nv; i += NT) {
