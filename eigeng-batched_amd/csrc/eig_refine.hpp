@@ -163,6 +163,7 @@ struct RfLds {
   int ord[256];
   int csz[256];
   int mlist[128];
+  unsigned char jn[256];                        // order positions q joined to q - 1
   double slab[RF_KMAX * RF_KMAX];               // the cluster's k x k scratch
   double cs[RF_KMAX * RF_KMAX], cg[RF_KMAX * RF_KMAX];   // second-order sums of S', G'
   double pr[RF_THREADS / 64][16];               // per-wave partial sums of a batch
@@ -466,21 +467,26 @@ __global__ __launch_bounds__(RF_THREADS) void eig_oa_stats_kernel(int L, int n, 
       ri.ord[r] = t;
     }
     __syncthreads();
+    // adjacent indices of the order join when their gap is within delta or
+    // their first-order correction would exceed RF_TAU: one pair per thread,
+    // so that the entries of S and G it may read load in parallel
+    if (t >= 1 && t < n) {
+      const int ia = sh.ord[t - 1], ib = sh.ord[t];
+      const double la = sh.lam[ia], lb = sh.lam[ib], gap = lb - la;
+      bool jn = gap <= delta;
+      if (!jn) {
+        const double s = rf_sym(Sb, n, ia, ib), g = rf_sym(Gb, n, ia, ib);
+        jn = fmax(fabs(s - lb * g), fabs(s - la * g)) > RF_TAU * gap;
+      }
+      sh.jn[t] = jn ? 1 : 0;
+    }
+    __syncthreads();
     if (t == 0) {
       int why_b = anybad ? RF_WHY_NAN : 0, nres = 0;
-      // adjacent indices join when their gap is within delta or their
-      // first-order correction would exceed RF_TAU
-      auto joins = [&](int q) -> bool {
-        const int ia = sh.ord[q - 1], ib = sh.ord[q];
-        const double la = sh.lam[ia], lb = sh.lam[ib], gap = lb - la;
-        if (gap <= delta) return true;
-        const double s = rf_sym(Sb, n, ia, ib), g = rf_sym(Gb, n, ia, ib);
-        return fmax(fabs(s - lb * g), fabs(s - la * g)) > RF_TAU * gap;
-      };
       int p = 0;
       while (p < n) {
         int e = p + 1;
-        while (e < n && joins(e)) ++e;
+        while (e < n && sh.jn[e]) ++e;
         const int k = e - p;
         for (int q = p; q < e; ++q) {
           const int i = sh.ord[q];
