@@ -327,7 +327,7 @@ def bench_n32(steps, warmup, tuned=True):
             "launch": {k: p[k] for k in ("grpSZ", "numTB", "numGR", "numTH")},
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": gbs / PEAK_HBM_GBS, "bytes_per_matrix": b,
-                         "kernel": "eig_wave_kernel<double,32> (one launch per step)",
+                         "kernel": "split path: eig_tri_kernel<double,32> + eig_ql_reg_kernel + eig_apply_kernel",
                          "valu_tflops": tfs, "valu_frac_fp64": tfs / PEAK_FP64_TFLOPS},
             "tuned": tl}
 

@@ -42,7 +42,7 @@ bool bad_args(int L, int nm, int n, int m) { return L < 1 || nm < n || n < 1 || 
 
 constexpr int WS_HEAD = 256;      // bytes; keeps the workspace non-empty and aligned
 constexpr int LCHUNK = 16384;     // split path: matrices per chunk (bounds the workspace)
-constexpr int SPLIT_MIN_L = 2048; // split path used from this batch size on (n <= 64)
+constexpr int SPLIT_MIN_L = 2048; // split path used from this batch size on (n <= 64; split_min_l)
 // solver tuning flags (_x entry points): force the fused or the split n <= 64
 // path; FLAG_TINYLOG shrinks the split path's rotation log so that every
 // matrix takes the overflow fallback (a host-forced test of that path)
@@ -125,13 +125,31 @@ struct SplitLargeLayout {
   }
 };
 
-bool use_split(int L, int n, int solver)
+// Smallest batch for which the split path (tridiagonalisation, lane-per-matrix
+// QL, replay) is the default for n <= 64 rather than the fused one-kernel
+// path.  Measured on one MI355X, random symmetric matrices, L = 128..2048
+// (split / fused ms): FP64 n = 16..32 the split path wins at every batch size
+// (n = 32: 0.36 / 0.42 at L = 128, 0.38 / 0.48 at L = 1024; n = 20 within 2 %
+// either way), FP32 from n = 28 (n = 32: 0.28 / 0.30 at L = 128; n = 20 and
+// 24 the fused kernel by 3-16 %); n = 8 and 12 the fused kernel wins up to
+// L = 2048 (FP64 n = 12: 0.110 / 0.105 at L = 2048); above n = 32 the fused
+// kernel wins on small batches, up to L = 1024 for FP32 n >= 56 (n = 64:
+// 0.92 / 0.97 at L = 1024) and to L = 2048 otherwise (FP64 n = 64: 1.35 / 1.20
+// at L = 1024, 1.43 / 2.33 at L = 2048).
+int split_min_l(int n, int tbytes)
+{
+  if (n <= 32 && n >= (tbytes == 8 ? 16 : 28)) return 1;
+  if (tbytes == 4 && n >= 56) return 1024;
+  return SPLIT_MIN_L;
+}
+
+bool use_split(int L, int n, int tbytes, int solver)
 {
   const int base = solver & 0xff, flags = solver & ~0xff;
   if (n > 64 || (base != SOLVER_IMTQL2 && base != SOLVER_TQL2)) return false;
   if (flags & FLAG_FUSED) return false;
   if (flags & FLAG_SPLIT) return true;
-  return L >= SPLIT_MIN_L;
+  return L >= split_min_l(n, tbytes);
 }
 
 // Which kernel serves (n, element size): register-resident groups while a row
@@ -232,7 +250,7 @@ size_t workspace_bytes(int L, int nm, int n, int m, int tbytes, int solver)
     if (tbytes != 8 || n > 256) return 0;
     return WS_HEAD + MixedLayout(L, n, mixed_wy(n, solver), solver & (3 << CHUNK_SHIFT)).total;
   }
-  if (use_split(L, n, solver) || use_split128(L, n, tbytes, solver))
+  if (use_split(L, n, tbytes, solver) || use_split128(L, n, tbytes, solver))
     return WS_HEAD + SplitLayout(L < lchunk(solver) ? L : lchunk(solver), n, tbytes).total;
   if (n > GL_NMAX) return 0;
   if (use_split_large(L, n, tbytes, solver))
@@ -807,13 +825,13 @@ hipError_t run(int L, int nm, int n, int m, T *a, T *w, void *wk, hipStream_t s,
   if (wk == nullptr) return hipErrorInvalidValue;
   // kernel options: solver, order of the output, eigenvector columns not stored
   const int kop = kopt_make(base, (solver & FLAG_UNSORTED) != 0, n - m);
-  if ((solver & FLAG_PH_ALL) && !use_split(L, n, solver) && !use_split_large(L, n, (int)sizeof(T), solver) &&
+  if ((solver & FLAG_PH_ALL) && !use_split(L, n, (int)sizeof(T), solver) && !use_split_large(L, n, (int)sizeof(T), solver) &&
       !use_split128(L, n, (int)sizeof(T), solver))
     return hipErrorInvalidValue;
   if constexpr (sizeof(T) == 4) {
     if (use_split128(L, n, (int)sizeof(T), solver)) return launch_split<T, 128>(L, nm, n, a, w, info, solver, kop, wk, s);
   }
-  if (use_split(L, n, solver)) {
+  if (use_split(L, n, (int)sizeof(T), solver)) {
     if (n <= 8) return launch_split<T, 8>(L, nm, n, a, w, info, solver, kop, wk, s);
     if (n <= 16) return launch_split<T, 16>(L, nm, n, a, w, info, solver, kop, wk, s);
     if (n <= 32) return launch_split<T, 32>(L, nm, n, a, w, info, solver, kop, wk, s);
@@ -1593,7 +1611,7 @@ hipError_t launch_params(int L, int n, int tb, int solver, int *p)
   };
   if (tb == 4 && use_split128(L, n, tb, solver)) {
     grp(128, L < lchunk(solver) ? L : lchunk(solver), 128);
-  } else if (use_split(L, n, solver)) {
+  } else if (use_split(L, n, tb, solver)) {
     grp(nb_of(n), L < lchunk(solver) ? L : lchunk(solver), 64);
   } else if (base == SOLVER_JACOBI) {
     if (n > 64) return hipErrorNotSupported;

@@ -119,11 +119,18 @@ def test_launch_params_follow_the_dispatch():
     """eigen_GPU_batch_launch_params reports the first kernel of the path a
     call takes: the fused register kernel for small batches (groups of 8..64
     lanes, 64 / group-size matrices per wave), the split path from 2048
-    matrices, the two-wave FP32 kernels up to n = 128, the workgroup kernels
-    beyond."""
+    matrices (from one matrix for FP64 16 <= n <= 32 and FP32 28 <= n <= 32,
+    from 1024 for FP32 n >= 56), the two-wave FP32
+    kernels up to n = 128, the workgroup kernels beyond."""
     eg.tune_reset()
-    p = eg.launch_params(1024, 32, 8)          # n = 32 FP64 L = 1024: fused, 2 matrices per wave
+    p = eg.launch_params(1024, 32, 8)          # n = 32 FP64 L = 1024: split, 2 matrices per wave
     assert p["rc"] == 0 and (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (32, 512, 2, 64)
+    p = eg.launch_params(1024, 48, 8)          # n = 48 FP64 L = 1024: fused, one matrix per wave
+    assert p["rc"] == 0 and (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (64, 1024, 1, 64)
+    assert eg.bufsize(1024, 32, 32, 32, 8) > eg.bufsize(1024, 48, 48, 48, 8)   # only split needs a log
+    p = eg.launch_params(1024, 8, 8)           # n = 8: fused below 2048 matrices
+    assert p["rc"] == 0 and (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (8, 128, 8, 64)
+    assert eg.bufsize(1024, 8, 8, 8, 8) < eg.bufsize(2048, 8, 8, 8, 8)
     p = eg.launch_params(100, 5, 4)
     assert (p["grpSZ"], p["numTB"], p["numGR"], p["numTH"]) == (8, 13, 8, 64)
     p = eg.launch_params(8192, 128, 4)         # FP32 n = 128, L >= 2048: two-wave register split
