@@ -868,8 +868,8 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only: matrices are independent, "
-                                   "a multi-GPU run is L/N contiguous slices with no exchange "
-                                   "(see DESIGN.md); --gpus must be 1"}))
+                                   "a multi-GPU run is L/N contiguous slices with no exchange; "
+                                   "--gpus must be 1"}))
         sys.exit(2)
 
     import numpy as np

@@ -10,7 +10,7 @@ Adam, on eigenpairs the plain solver produces for the first steps of the stream.
 
 Everything here drives the HIP library through its C ABI; torch only holds
 device memory and streams.  The reference has no learned component: this is
-the warm-start part of the north-star design, built around the reference's
+a warm start for a stream of related batches, built around the reference's
 Jacobi path (jac_tiled.hpp:3-152), which always starts from the identity.
 """
 from __future__ import annotations
