@@ -26,10 +26,10 @@
 //
 // Shape: a 256-thread workgroup computes a 64 x 64 tile of one C_b; wave q owns
 // the 32 x 32 quarter (q & 1, q >> 1), i.e. 2 x 2 MFMA tiles of 16 x 16.  The
-// contraction runs in chunks of 16: the next chunk's global loads are issued
-// before the current chunk's MFMAs, then stored to LDS (k-major, so that the
-// lanes of an MFMA operand read consecutive words), into one of two LDS buffers
-// while the other one is multiplied.  Out-of-range rows, columns
+// contraction runs in chunks of 16: a chunk's global loads are issued two
+// chunks ahead, into one of two register stages, and stored to LDS (k-major, so
+// that the lanes of an MFMA operand read consecutive words) one chunk ahead,
+// into one of two LDS buffers while the other one is multiplied.  Out-of-range rows, columns
 // and k are zero-filled, so any M, N, K >= 1 is valid.
 //
 // sym > 0: C_b is a row of `sym`-wide square blocks of tiles whose results are
@@ -302,22 +302,36 @@ __global__ __launch_bounds__(GM_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     if (!g.transb) { bk[i] = tid & 15; bc[i] = (tid >> 4) + 16 * i; }
     else { bc[i] = tid & 63; bk[i] = (tid >> 6) + 4 * i; }
   }
+  // every load is issued, from indices clamped into the operand, and
+  // out-of-range elements become zero when the chunk is stored to LDS: with no
+  // branch or select after them the loads of the next chunks stay in flight
+  // across this chunk's products
+  const long long lsa_r = g.transa ? g.lda : 1, lsa_k = g.transa ? 1 : g.lda;
+  const long long lsb_c = g.transb ? 1 : g.ldb, lsb_k = g.transb ? g.ldb : 1;
   auto loadA = [&](int k0, S (&ra)[4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = row0 + ar[i], k = k0 + ak[i];
-      S v = S(0);
-      if (r < g.M && k < g.K) v = g.transa ? A[k + (size_t)r * g.lda] : A[r + (size_t)k * g.lda];
-      ra[i] = v;
+      const int rc = r < g.M ? r : g.M - 1, kc = k < g.K ? k : g.K - 1;
+      ra[i] = A[rc * lsa_r + kc * lsa_k];
     }
   };
   auto loadB = [&](int k0, S (&rb)[4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int c = col0 + bc[i], k = k0 + bk[i];
-      S v = S(0);
-      if (c < g.N && k < g.K) v = g.transb ? B[c + (size_t)k * g.ldb] : B[k + (size_t)c * g.ldb];
-      rb[i] = v;
+      const int cc = c < g.N ? c : g.N - 1, kc = k < g.K ? k : g.K - 1;
+      rb[i] = B[cc * lsb_c + kc * lsb_k];
+    }
+  };
+  auto store = [&](int k0, const S (&ra)[4], const S (&rb)[4], Ls (*Ad)[GM_TILE + GM_PAD],
+                   Ls (*Bd)[GM_TILE + GM_PAD]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool ina = row0 + ar[i] < g.M && k0 + ak[i] < g.K;
+      const bool inb = col0 + bc[i] < g.N && k0 + bk[i] < g.K;
+      Ad[ak[i]][ar[i]] = TT::cvt(ina ? ra[i] : S(0));
+      Bd[bk[i]][bc[i]] = TT::cvt(inb ? rb[i] : S(0));
     }
   };
 
@@ -329,35 +343,36 @@ __global__ __launch_bounds__(GM_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[x][y][r] = acc2[x][y][r] = 0;
 
-  S ra[4], rb[4];
-  loadA(0, ra);
-  loadB(0, rb);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    As[0][ak[i]][ar[i]] = TT::cvt(ra[i]);
-    Bs[0][bk[i]][bc[i]] = TT::cvt(rb[i]);
-  }
+  // two register stages: while chunk c is multiplied from LDS, chunk c + 1
+  // (loaded during chunk c - 1) goes from registers into the other LDS buffer
+  // and chunk c + 2 is loaded, so each global load has two chunks of products
+  // to arrive in.  The loop body has no branch: loads past K read clamped
+  // indices and stores past K write zeros (an odd chunk count multiplies one
+  // zero chunk), which keeps the wait for a stage's loads from also waiting
+  // for the other stage's
+  S ra0[4], rb0[4], ra1[4], rb1[4];
+  loadA(0, ra0);
+  loadB(0, rb0);
+  store(0, ra0, rb0, As[0], Bs[0]);
+  loadA(GM_KC, ra1);
+  loadB(GM_KC, rb1);
   __syncthreads();
   int cur = 0;
-  for (int k0 = 0; k0 < g.K; k0 += GM_KC) {
-    // chunk k0 + GM_KC: global loads before, LDS stores after this chunk's
-    // products; the buffer it goes to was last read before the previous barrier
-    const bool more = k0 + GM_KC < g.K;
-    if (more) {
-      loadA(k0 + GM_KC, ra);
-      loadB(k0 + GM_KC, rb);
-    }
+  // chunk k0 in LDS buffer cur, chunk k0 + GM_KC in (sa, sb), chunk k0 + 2 GM_KC
+  // loaded into (la, lb), whose previous chunk is already in LDS
+  auto step = [&](int k0, S (&la)[4], S (&lb)[4], const S (&sa)[4], const S (&sb)[4]) {
+    loadA(k0 + 2 * GM_KC, la);
+    loadB(k0 + 2 * GM_KC, lb);
     if constexpr (KIND == GK_F16X3) GemmMma<KIND>::chunk(As[cur], Bs[cur], wr, wc, m, gg, acc, acc2);
     else GemmMma<KIND>::chunk(As[cur], Bs[cur], wr, wc, m, gg, acc);
-    if (more) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        As[cur ^ 1][ak[i]][ar[i]] = TT::cvt(ra[i]);
-        Bs[cur ^ 1][bk[i]][bc[i]] = TT::cvt(rb[i]);
-      }
-    }
+    // the buffer written here was last read before the previous barrier
+    store(k0 + GM_KC, sa, sb, As[cur ^ 1], Bs[cur ^ 1]);
     __syncthreads();
     cur ^= 1;
+  };
+  for (int k0 = 0; k0 < g.K; k0 += 2 * GM_KC) {
+    step(k0, ra0, rb0, ra1, rb1);
+    step(k0 + GM_KC, ra1, rb1, ra0, rb0);
   }
 
   // epilogue: C = alpha acc + beta D
