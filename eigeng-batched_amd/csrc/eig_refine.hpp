@@ -187,7 +187,7 @@ __device__ __forceinline__ double rf_wave_max(double v)
 // element (i, j) of a symmetric matrix whose upper triangle is stored
 __device__ __forceinline__ double rf_sym(const double *__restrict__ X, int n, int i, int j)
 {
-  return i <= j ? X[i + (size_t)j * n] : X[j + (size_t)i * n];
+  return X[(size_t)min(i, j) + (size_t)max(i, j) * n];   // the upper triangle; one load, no branch
 }
 
 // Rayleigh-Ritz on the cluster J at sorted positions p0 .. p0+k-1 (one thread).
@@ -362,11 +362,14 @@ __device__ void rf_cluster_sums(RfLds &sh, int p0, const double *__restrict__ Sb
 #pragma unroll
   for (int a = 0; a < K; ++a) use = use && idx[a] != t;
   const double li = use ? sh.lam[t] : 0.0;
+  // every thread loads (row 0 beyond n) and masks: the 2 K loads are in flight together
+  const int tc = t < n ? t : 0;
   double si[K], gi[K], f[K];
 #pragma unroll
   for (int a = 0; a < K; ++a) {
-    si[a] = use ? rf_sym(Sb, n, t, idx[a]) : 0.0;
-    gi[a] = use ? rf_sym(Gb, n, t, idx[a]) : 0.0;
+    const double sv = rf_sym(Sb, n, tc, idx[a]), gv = rf_sym(Gb, n, tc, idx[a]);
+    si[a] = use ? sv : 0.0;
+    gi[a] = use ? gv : 0.0;
     const double la = sh.lam[idx[a]];
     f[a] = use ? (si[a] - la * gi[a]) / (la - li) : 0.0;
   }
@@ -435,14 +438,18 @@ __global__ __launch_bounds__(RF_THREADS) void eig_oa_stats_kernel(int L, int n, 
       mx = fabs(l);
       bad = !isfinite(l);
     }
-    // strict upper triangle: column j, rows t < j (coalesced, independent loads)
-#pragma unroll 8
-    for (int j = 1; j < n; ++j)
-      if (t < j) {
-        const double s = Sb[t + (size_t)j * n], g = Gb[t + (size_t)j * n];
-        so = fma(2.0 * s, s, so);
-        go = fma(2.0 * g, g, go);
-      }
+    // strict upper triangle: column j, rows t < j (coalesced, independent
+    // loads).  Every thread loads (row 0 where t >= j) and masks the value, so
+    // that no branch separates the loads of the unrolled columns and they are
+    // all in flight before the first is used
+#pragma unroll 16
+    for (int j = 1; j < n; ++j) {
+      const bool in = t < j;
+      const size_t o = (size_t)(in ? t : 0) + (size_t)j * n;
+      const double s = Sb[o], g = Gb[o];
+      so = fma(2.0 * s, in ? s : 0.0, so);
+      go = fma(2.0 * g, in ? g : 0.0, go);
+    }
     so = rf_wave_sum(so);
     go = rf_wave_sum(go);
     mx = rf_wave_max(mx);
@@ -557,12 +564,16 @@ __global__ __launch_bounds__(256) void eig_oa_fill_kernel(int L, int n, const do
     const double *__restrict__ Sb = S + b * nn;
     const double *__restrict__ Gb = G + b * nn;
     const RfInfo ri = rf_info_at(inf, lc, n, b);
-#pragma unroll 4
+    // every thread loads (indices clamped into the matrix) and masks the
+    // value: no branch between the loads, so they are all in flight at once
+#pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int cc = cq + 4 * k, i = lr0 + rr, j = lc0 + cc;
       const bool in = i < n && j < n;
-      Ts[rr][cc] = in ? Sb[i + (size_t)j * n] : 0.0;
-      Tg[rr][cc] = in ? Gb[i + (size_t)j * n] : 0.0;
+      const size_t o = (size_t)min(i, n - 1) + (size_t)min(j, n - 1) * n;
+      const double sv = Sb[o], gv = Gb[o];
+      Ts[rr][cc] = in ? sv : 0.0;
+      Tg[rr][cc] = in ? gv : 0.0;
     }
     if (threadIdx.x < 64) {
       const int i = I * 64 + rr;
