@@ -670,6 +670,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ApplyWaves<T
   T R[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) R[j] = (j < n && k < n) ? Q[k + (size_t)j * n] : T(0);
+  // the row's loads complete here, after all of them were issued: left
+  // pending into the sweep loop, the wait for them inside the loop would also
+  // wait for the next sweep's prefetched rotations, on every sweep
+#pragma unroll
+  for (int j = 0; j < NB; ++j) asm volatile("" : "+v"(R[j]));
   if constexpr (ApplyLds<T, NB>::NS == 2) {
 #pragma unroll
     for (int t = 0; t < NR; ++t) {
