@@ -81,7 +81,9 @@ enum eigen_GPU_batch_solver {
 
 /*
  * Tuning bits OR-ed into `solver` of the _x entry points and of
- * eigen_GPU_batch_bufsize (the default picks by n and L):
+ * eigen_GPU_batch_bufsize (the default picks by n, precision and L: for n <= 64
+ * the split path from 2048 matrices, from one matrix for FP64 16 <= n <= 32 and
+ * FP32 28 <= n <= 32, from 1024 for FP32 n >= 56, the fused kernel below):
  *   0x100  one fused kernel per matrix group;
  *   0x200  split n <= 64 path: register tridiagonalisation, one-lane-per-matrix
  *          QL recurrence writing a rotation log, rotation apply kernel;
